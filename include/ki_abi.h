@@ -1,0 +1,161 @@
+/*
+ * ki_abi.h -- C-ABI of the MI355X recursive-skeletonization solver.
+ *
+ * This is the drop-in boundary for the reference's solve path
+ * (jpryan1/kern-interp-dev).  Plain C types only: pointers, sizes, status
+ * codes.  Each entry point names the reference interface it replaces
+ * (paths relative to the reference repository).  The C++ facade in
+ * include/kern_interp_amd.hpp re-creates the reference classes on top of it.
+ *
+ * Threading: a context is used by one host thread at a time.  Every call is
+ * synchronous with respect to the host (results are ready on return).
+ * Errors: every function returns KI_OK (0) or a negative KI_E_* code and
+ * records a message readable with ki_last_error() (thread-local).
+ */
+#ifndef KI_ABI_H_
+#define KI_ABI_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  KI_OK = 0,
+  KI_E_ARG = -1,        /* bad argument / shape (reference: assert)       */
+  KI_E_STATE = -2,      /* call out of order (e.g. solve before factor)   */
+  KI_E_DEVICE = -3,     /* HIP runtime error                              */
+  KI_E_NUMERIC = -4,    /* non-finite data, failed convergence            */
+  KI_E_NOMEM = -5       /* device allocation failed                       */
+};
+
+/* PDE codes: the reference's Kernel::Pde values (kernel/kernel.h:19-24). */
+enum { KI_LAPLACE = 0, KI_STOKES = 3 };
+
+typedef struct ki_ctx ki_ctx;
+
+/* Solver parameters.  The reference keeps these as compile-time constants
+ * (skel_factorization.h:12, quadtree.h:11, kernel.h:13-14) plus the
+ * SkelFactorization(id_tol, ...) constructor argument (skel_factorization.h:27);
+ * ki_params_default() returns the reference values. */
+typedef struct ki_params {
+  double id_tol;            /* relative ID tolerance on |R_ii| / |R_00|        */
+  int min_dofs_to_compress; /* MIN_DOFS_TO_COMPRESS (64)                       */
+  int proxy_points;         /* 2D: circle points; 3D: total sphere points      */
+  double proxy_ratio;       /* proxy radius = ratio * sqrt(d) * side / 2 (2.0) */
+  int proxy_kind;           /* 0: reference rule (2D circle, 3D GL theta x phi)
+                               1: Fibonacci sphere (3D)                        */
+  int precision;            /* 64: plain FP64; 32: FP32 fill/ID/Schur/sweeps
+                               with FP64 top-level LU                          */
+  int qr_full;              /* 1: run every CPQR step like dgeqp3; 0: stop at
+                               the step where the rank rule fires              */
+  int use_predictor;        /* 1: size each level from the learned rank model  */
+  int reserved[8];
+} ki_params;
+
+void ki_params_default(ki_params* p);
+
+/* Last error message of the calling thread ("" when none). */
+const char* ki_last_error(void);
+
+/* Context on one HIP device; owns geometry, tree and factorization. */
+int ki_ctx_create(int device, ki_ctx** out);
+void ki_ctx_destroy(ki_ctx* ctx);
+
+/* Boundary data; replaces Kernel(sol, dom, pde, Boundary*, ...) and
+ * Kernel::update_data (kernel/kernel.h:38-66, kernel.cpp:24-29).
+ * pts/nrm: npts*dom point-major; w, curv: npts (curv may be NULL in 3D). */
+int ki_set_geometry(ki_ctx* ctx, int dom, int pde, const double* pts,
+                    const double* nrm, const double* w, const double* curv,
+                    int npts);
+
+/* 3D row-sum diagonals; replaces Kernel::compute_diag_entries_3dlaplace /
+ * _3dstokes (kernel.cpp:124-301) for a boundary without holes. */
+int ki_compute_diag3d(ki_ctx* ctx);
+/* Copy the diagonals out: npts (Laplace) or npts*9 (Stokes, row-major 3x3). */
+int ki_get_diag3d(ki_ctx* ctx, double* out, int64_t count);
+
+/* Tree build; replaces QuadTree::initialize_tree (quadtree.cpp:240-287),
+ * with MAX_LEAF_DOFS (quadtree.h:11) as an argument. */
+int ki_build_tree(ki_ctx* ctx, int max_leaf_dofs);
+int ki_tree_num_levels(ki_ctx* ctx);
+int ki_tree_level_size(ki_ctx* ctx, int level);
+/* info[0..7] = is_leaf, n_original, n_active, n_neighbors, compressed, k,
+ * n_near, parent index within its level (-1 at the root);
+ * geom[0..3] = centre (3), side. */
+int ki_tree_box(ki_ctx* ctx, int level, int idx, int* info, double* geom);
+/* which: 0 original_box, 1 active_box, 2 skel, 3 redundant, 4 near,
+ * 5 neighbors (as level*2^20 + index pairs).  Returns KI_OK and the full
+ * length in *count; copies min(cap, count) entries. */
+int ki_tree_box_list(ki_ctx* ctx, int level, int idx, int which, int* out,
+                     int cap, int* count);
+
+/* Factorization; replaces SkelFactorization(id_tol, ...)::skeletonize
+ * (skel_factorization.cpp:173-310). */
+int ki_skeletonize(ki_ctx* ctx, const ki_params* params);
+
+/* x = A^{-1} b with the factorization, host arrays of length N_dof;
+ * replaces SkelFactorization::solve (skel_factorization.cpp:595-778). */
+int ki_solve(ki_ctx* ctx, const double* b, double* x);
+
+/* y = K mu (FP64, matrix-free, host arrays); the operator the reference
+ * forms densely as kernel(all, all) in solve_err (linear_solve.cpp:407-424). */
+int ki_matvec(ki_ctx* ctx, const double* mu, double* y);
+
+/* FP64 GMRES on K mu = b, right-preconditioned by the factorization, to a
+ * relative residual <= tol (host arrays).  No reference counterpart: the
+ * reference stops at the direct solve. */
+int ki_gmres(ki_ctx* ctx, const double* b, double* x, double tol, int maxit,
+             int restart, int* iters, double* relres);
+
+/* Device-resident variants for throughput runs: ki_set_rhs uploads b once;
+ * ki_gmres_resident solves into a device vector that ki_get_solution copies
+ * back. */
+int ki_set_rhs(ki_ctx* ctx, const double* b);
+int ki_gmres_resident(ki_ctx* ctx, double tol, int maxit, int restart,
+                      int* iters, double* relres);
+int ki_get_solution(ki_ctx* ctx, double* x);
+
+/* K(tgt, src) as a dense column-major block (FP64); replaces
+ * Kernel::operator() / get_3d (kernel.cpp:304-429). */
+int ki_kernel_block(ki_ctx* ctx, const int* tgt, int nt, const int* src,
+                    int ns, double* out);
+
+/* Proxy rows [box->proxy ; proxy->box] for the DOFs `box` (column-major,
+ * *rows x n), proxy surface from `params` around `center` (3 doubles) at
+ * `radius`; replaces Kernel::get_proxy_mat / get_proxy_mat3d
+ * (kernel.cpp:646-796).  With out == NULL only *rows is set. */
+int ki_proxy_block(ki_ctx* ctx, const ki_params* params, const double* center,
+                   double radius, const int* box, int n, double* out,
+                   int* rows);
+
+/* Field at domain points: replaces Kernel::apply_forward (kernel.cpp:809-825).
+ * dpts: ndp*dom; mu: N_dof; out: ndp*sol. */
+int ki_apply_forward(ki_ctx* ctx, const double* dpts, int ndp,
+                     const double* mu, double* out);
+
+/* Factorization statistics. */
+typedef struct ki_stats {
+  int levels;
+  int top_size;             /* |allskel|                                     */
+  int boxes_compressed;
+  int boxes_fallback;       /* predictor under-shoots re-run as full CPQR    */
+  double t_total;           /* seconds, whole skeletonize                    */
+  double t_fill, t_qr, t_decouple, t_top;  /* seconds per phase (events)     */
+  double bytes_fill, bytes_qr, flops_schur, flops_top;  /* algorithmic       */
+  double t_solve_last;      /* seconds of the last factor apply              */
+  double t_matvec_last;     /* seconds of the last FP64 matvec               */
+  int64_t device_bytes;     /* device memory held by the factorization       */
+} ki_stats;
+int ki_get_stats(ki_ctx* ctx, ki_stats* out);
+
+/* Per compressed box: (level, n, m, k) quadruples in factorization order;
+ * returns the number of boxes in *count, copies min(cap, count). */
+int ki_box_log(ki_ctx* ctx, int* out, int cap, int* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KI_ABI_H_ */

@@ -1,0 +1,394 @@
+"""Python host mirror of the MI355X recursive-skeletonization solver.
+
+The native library ``libki_hip.so`` (HIP kernels + C-ABI, ``include/ki_abi.h``)
+does all the work; this module binds it with ctypes and re-creates the
+reference's operator interface for the solve path with the same names and
+argument meaning (``Kernel``, ``QuadTree``, ``SkelFactorization``,
+``boundary_integral_solve``; reference ``kernel/kernel.h``,
+``quadtree/quadtree.h``, ``skel_factorization/skel_factorization.h``,
+``linear_solve.h``).
+
+There is no CPU fallback: if the library is missing or no HIP device is
+present, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libki_hip.so"
+
+KI_LAPLACE, KI_STOKES = 0, 3
+
+# Names every build of the C-ABI must export (include/ki_abi.h).
+EXPORTS = [
+    "ki_params_default", "ki_last_error", "ki_ctx_create", "ki_ctx_destroy",
+    "ki_set_geometry", "ki_compute_diag3d", "ki_get_diag3d", "ki_build_tree",
+    "ki_tree_num_levels", "ki_tree_level_size", "ki_tree_box",
+    "ki_tree_box_list", "ki_skeletonize", "ki_solve", "ki_matvec", "ki_gmres",
+    "ki_set_rhs", "ki_gmres_resident", "ki_get_solution", "ki_kernel_block",
+    "ki_apply_forward", "ki_get_stats", "ki_box_log", "ki_proxy_block",
+]
+
+
+class KiParams(C.Structure):
+    _fields_ = [
+        ("id_tol", C.c_double),
+        ("min_dofs_to_compress", C.c_int),
+        ("proxy_points", C.c_int),
+        ("proxy_ratio", C.c_double),
+        ("proxy_kind", C.c_int),
+        ("precision", C.c_int),
+        ("qr_full", C.c_int),
+        ("use_predictor", C.c_int),
+        ("reserved", C.c_int * 8),
+    ]
+
+
+class KiStats(C.Structure):
+    _fields_ = [
+        ("levels", C.c_int),
+        ("top_size", C.c_int),
+        ("boxes_compressed", C.c_int),
+        ("boxes_fallback", C.c_int),
+        ("t_total", C.c_double),
+        ("t_fill", C.c_double),
+        ("t_qr", C.c_double),
+        ("t_decouple", C.c_double),
+        ("t_top", C.c_double),
+        ("bytes_fill", C.c_double),
+        ("bytes_qr", C.c_double),
+        ("flops_schur", C.c_double),
+        ("flops_top", C.c_double),
+        ("t_solve_last", C.c_double),
+        ("t_matvec_last", C.c_double),
+        ("device_bytes", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_lib = None
+
+
+def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
+    """Load libki_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError("native library %s is missing: run "
+                           "__graft_entry__.build() first" % p)
+    lib = C.CDLL(str(p))
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    vp = C.c_void_p
+    sig = {
+        "ki_params_default": (None, [C.POINTER(KiParams)]),
+        "ki_last_error": (C.c_char_p, []),
+        "ki_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+        "ki_ctx_destroy": (None, [vp]),
+        "ki_set_geometry": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp, dp,
+                                      C.c_int]),
+        "ki_compute_diag3d": (C.c_int, [vp]),
+        "ki_get_diag3d": (C.c_int, [vp, dp, C.c_int64]),
+        "ki_build_tree": (C.c_int, [vp, C.c_int]),
+        "ki_tree_num_levels": (C.c_int, [vp]),
+        "ki_tree_level_size": (C.c_int, [vp, C.c_int]),
+        "ki_tree_box": (C.c_int, [vp, C.c_int, C.c_int, ip, dp]),
+        "ki_tree_box_list": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, ip,
+                                       C.c_int, ip]),
+        "ki_skeletonize": (C.c_int, [vp, C.POINTER(KiParams)]),
+        "ki_solve": (C.c_int, [vp, dp, dp]),
+        "ki_matvec": (C.c_int, [vp, dp, dp]),
+        "ki_gmres": (C.c_int, [vp, dp, dp, C.c_double, C.c_int, C.c_int, ip,
+                               dp]),
+        "ki_set_rhs": (C.c_int, [vp, dp]),
+        "ki_gmres_resident": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, ip,
+                                        dp]),
+        "ki_get_solution": (C.c_int, [vp, dp]),
+        "ki_kernel_block": (C.c_int, [vp, ip, C.c_int, ip, C.c_int, dp]),
+        "ki_apply_forward": (C.c_int, [vp, dp, C.c_int, dp, dp]),
+        "ki_get_stats": (C.c_int, [vp, C.POINTER(KiStats)]),
+        "ki_box_log": (C.c_int, [vp, ip, C.c_int, ip]),
+        "ki_proxy_block": (C.c_int, [vp, C.POINTER(KiParams), dp, C.c_double,
+                                     ip, C.c_int, dp, ip]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+class KiError(RuntimeError):
+    pass
+
+
+def _check(rc: int):
+    if rc != 0:
+        msg = _lib.ki_last_error().decode()
+        raise KiError("ki error %d: %s" % (rc, msg))
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _iptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def default_params(**kw) -> KiParams:
+    lib = load_library()
+    p = KiParams()
+    lib.ki_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class Context:
+    """One device context: geometry, tree and factorization (C-ABI ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        _check(self.lib.ki_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.dom = self.sol = self.npts = 0
+
+    def close(self):
+        if self.h:
+            self.lib.ki_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- geometry / kernel --------------------------------------------------
+    def set_geometry(self, boundary, pde: int):
+        b = boundary
+        self.dom = b.dom
+        self.sol = 1 if pde == KI_LAPLACE else b.dom
+        self.npts = b.npts
+        self._pts = np.ascontiguousarray(b.points, dtype=np.float64)
+        self._nrm = np.ascontiguousarray(b.normals, dtype=np.float64)
+        self._w = np.ascontiguousarray(b.weights, dtype=np.float64)
+        self._c = np.ascontiguousarray(b.curvatures, dtype=np.float64)
+        _check(self.lib.ki_set_geometry(self.h, b.dom, pde, _dptr(self._pts),
+                                        _dptr(self._nrm), _dptr(self._w),
+                                        _dptr(self._c), b.npts))
+        if b.dom == 3:
+            _check(self.lib.ki_compute_diag3d(self.h))
+
+    @property
+    def ndof(self) -> int:
+        return self.npts * self.sol
+
+    def diag3d(self) -> np.ndarray:
+        n = self.npts * (1 if self.sol == 1 else 9)
+        out = np.zeros(n)
+        _check(self.lib.ki_get_diag3d(self.h, _dptr(out), n))
+        return out if self.sol == 1 else out.reshape(-1, 3, 3)
+
+    def kernel_block(self, tgt, src) -> np.ndarray:
+        t = np.ascontiguousarray(tgt, dtype=np.int32)
+        s = np.ascontiguousarray(src, dtype=np.int32)
+        out = np.zeros((len(s), len(t)))  # column-major (nt x ns)
+        _check(self.lib.ki_kernel_block(self.h, _iptr(t), len(t), _iptr(s),
+                                        len(s), _dptr(out)))
+        return out.T.copy()
+
+    def proxy_block(self, params: KiParams, center, radius, box) -> np.ndarray:
+        b = np.ascontiguousarray(box, dtype=np.int32)
+        c = np.zeros(3)
+        c[:len(center)] = center
+        rows = C.c_int()
+        _check(self.lib.ki_proxy_block(self.h, C.byref(params), _dptr(c),
+                                       radius, _iptr(b), len(b), None,
+                                       C.byref(rows)))
+        out = np.zeros((len(b), rows.value))
+        _check(self.lib.ki_proxy_block(self.h, C.byref(params), _dptr(c),
+                                       radius, _iptr(b), len(b), _dptr(out),
+                                       C.byref(rows)))
+        return out.T.copy()
+
+    def matvec(self, mu) -> np.ndarray:
+        mu = np.ascontiguousarray(mu, dtype=np.float64)
+        y = np.zeros(self.ndof)
+        _check(self.lib.ki_matvec(self.h, _dptr(mu), _dptr(y)))
+        return y
+
+    def apply_forward(self, domain_points, mu) -> np.ndarray:
+        dp = np.ascontiguousarray(domain_points, dtype=np.float64).ravel()
+        mu = np.ascontiguousarray(mu, dtype=np.float64)
+        nd = dp.size // self.dom
+        out = np.zeros(nd * self.sol)
+        _check(self.lib.ki_apply_forward(self.h, _dptr(dp), nd, _dptr(mu),
+                                         _dptr(out)))
+        return out
+
+    # -- tree -----------------------------------------------------------------
+    def build_tree(self, max_leaf_dofs: int):
+        _check(self.lib.ki_build_tree(self.h, max_leaf_dofs))
+
+    def num_levels(self) -> int:
+        return self.lib.ki_tree_num_levels(self.h)
+
+    def level_size(self, level: int) -> int:
+        return self.lib.ki_tree_level_size(self.h, level)
+
+    def box(self, level: int, idx: int):
+        info = np.zeros(8, dtype=np.int32)
+        geom = np.zeros(4)
+        _check(self.lib.ki_tree_box(self.h, level, idx, _iptr(info),
+                                    _dptr(geom)))
+        return {"is_leaf": bool(info[0]), "n_original": int(info[1]),
+                "n_active": int(info[2]), "n_neighbors": int(info[3]),
+                "compressed": bool(info[4]), "k": int(info[5]),
+                "n_near": int(info[6]), "parent": int(info[7]),
+                "center": geom[:3].copy(), "side": float(geom[3])}
+
+    def box_list(self, level: int, idx: int, which: str) -> np.ndarray:
+        sel = {"original": 0, "active": 1, "skel": 2, "redundant": 3,
+               "near": 4, "neighbors": 5}[which]
+        cnt = C.c_int()
+        _check(self.lib.ki_tree_box_list(self.h, level, idx, sel, None, 0,
+                                         C.byref(cnt)))
+        out = np.zeros(max(cnt.value, 1), dtype=np.int32)
+        _check(self.lib.ki_tree_box_list(self.h, level, idx, sel, _iptr(out),
+                                         cnt.value, C.byref(cnt)))
+        return out[:cnt.value]
+
+    # -- factorization --------------------------------------------------------
+    def skeletonize(self, params: KiParams):
+        _check(self.lib.ki_skeletonize(self.h, C.byref(params)))
+
+    def solve(self, b) -> np.ndarray:
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.ndof)
+        _check(self.lib.ki_solve(self.h, _dptr(b), _dptr(x)))
+        return x
+
+    def gmres(self, b, tol=1e-10, maxit=50, restart=20):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.ndof)
+        it = C.c_int()
+        rel = C.c_double()
+        _check(self.lib.ki_gmres(self.h, _dptr(b), _dptr(x), tol, maxit,
+                                 restart, C.byref(it), C.byref(rel)))
+        return x, it.value, rel.value
+
+    def set_rhs(self, b):
+        self._rhs = np.ascontiguousarray(b, dtype=np.float64)
+        _check(self.lib.ki_set_rhs(self.h, _dptr(self._rhs)))
+
+    def gmres_resident(self, tol=1e-10, maxit=50, restart=20):
+        it = C.c_int()
+        rel = C.c_double()
+        _check(self.lib.ki_gmres_resident(self.h, tol, maxit, restart,
+                                          C.byref(it), C.byref(rel)))
+        return it.value, rel.value
+
+    def solution(self) -> np.ndarray:
+        x = np.zeros(self.ndof)
+        _check(self.lib.ki_get_solution(self.h, _dptr(x)))
+        return x
+
+    def stats(self) -> dict:
+        st = KiStats()
+        _check(self.lib.ki_get_stats(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def box_log(self) -> np.ndarray:
+        cnt = C.c_int()
+        _check(self.lib.ki_box_log(self.h, None, 0, C.byref(cnt)))
+        out = np.zeros((max(cnt.value, 1), 4), dtype=np.int32)
+        _check(self.lib.ki_box_log(self.h, _iptr(out), cnt.value,
+                                   C.byref(cnt)))
+        return out[:cnt.value]
+
+
+# ---------------------------------------------------------------------------
+# Reference-named interface (kernel.h, quadtree.h, skel_factorization.h)
+# ---------------------------------------------------------------------------
+
+class Kernel:
+    """Kernel(solution_dimension, domain_dimension, pde, boundary)."""
+    LAPLACE, STOKES = KI_LAPLACE, KI_STOKES
+
+    def __init__(self, solution_dimension, domain_dimension, pde, boundary,
+                 domain_points=None, device=0):
+        self.solution_dimension = solution_dimension
+        self.domain_dimension = domain_dimension
+        self.pde = pde
+        self.domain_points = domain_points
+        self.ctx = Context(device)
+        self.ctx.set_geometry(boundary, pde)
+
+    def __call__(self, tgt_inds, src_inds):
+        return self.ctx.kernel_block(tgt_inds, src_inds)
+
+    def apply_forward(self, mu):
+        return self.ctx.apply_forward(self.domain_points, mu)
+
+
+class QuadTree:
+    """QuadTree bound to a Kernel's context (quadtree.h:111-179)."""
+
+    def __init__(self, kernel: Kernel):
+        self.kernel = kernel
+        self.ctx = kernel.ctx
+
+    def initialize_tree(self, boundary, solution_dimension, domain_dimension,
+                        max_leaf_dofs=64):
+        self.ctx.build_tree(max_leaf_dofs)
+
+    @property
+    def levels(self):
+        return [self.ctx.level_size(l) for l in range(self.ctx.num_levels())]
+
+
+class SkelFactorization:
+    """SkelFactorization(id_tol, fact_threads) (skel_factorization.h:27)."""
+
+    def __init__(self, id_tol, fact_threads=1, **params):
+        self.id_tol = id_tol
+        self.fact_threads = fact_threads  # accepted; the GPU path ignores it
+        self.params = default_params(id_tol=id_tol, **params)
+
+    def skeletonize(self, kernel: Kernel, tree: QuadTree):
+        tree.ctx.skeletonize(self.params)
+
+    def solve(self, tree: QuadTree, b):
+        return tree.ctx.solve(b)
+
+
+def boundary_integral_solve(kernel: Kernel, boundary, tree: QuadTree, id_tol,
+                            fact_threads=1, max_leaf_dofs=64, **params):
+    """linear_solve.cpp:257-303 minus the is_in_domain masking: factor,
+    solve, and evaluate the field at the kernel's domain points."""
+    tree.initialize_tree(boundary, kernel.solution_dimension,
+                         kernel.domain_dimension, max_leaf_dofs)
+    fac = SkelFactorization(id_tol, fact_threads, **params)
+    fac.skeletonize(kernel, tree)
+    mu = fac.solve(tree, _boundary_values(boundary))
+    return kernel.apply_forward(mu)
+
+
+def _boundary_values(boundary):
+    bv = getattr(boundary, "boundary_values", None)
+    if bv is None:
+        raise ValueError("boundary has no boundary_values")
+    return np.asarray(bv, dtype=np.float64)

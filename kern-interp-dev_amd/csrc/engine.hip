@@ -1,0 +1,1432 @@
+// Engine: the level loop of recursive skeletonization on one HIP device, the
+// factor apply, FP64 GMRES refinement and the C-ABI (include/ki_abi.h).
+//
+// Per tree level (deepest first), restating SkelFactorization::skeletonize
+// (skel_factorization.cpp:173-310):
+//   host   active / near lists, ID row sets (quadtree.cpp:1142-1204,
+//          kernel.cpp:432-641)
+//   GPU    fill every box's ID matrix in one launch
+//   GPU    batched CPQR + T = R11^-1 R12 (ki_mat.cpp:525-568)
+//   host   read ranks and pivots, form skel / redundant sets
+//   GPU    fill K(BN,BN) in skeleton-first order, subtract the children's
+//          accumulated Schur updates (scatter; quadtree.cpp:1545-1632)
+//   GPU    decouple (skel_factorization.cpp:91-127): 4 GEMMs, LU(X_rr),
+//          two LU solves, the Schur GEMM -- each batched over the level
+// then the top-level FP64 LU of K(allskel, allskel) minus updates.
+//
+// Accumulated updates.  The reference re-walks every compressed descendant
+// for each box (get_descendents_updates).  Here each box keeps Acc, the sum
+// of all descendant Schur complements restricted to its output DOFs (its
+// skeleton when compressed, else its active set); a parent's update is the
+// block-diagonal scatter of its children's Acc.  Restriction composes, so
+// the sum is the same; only the order of FP additions differs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "ki_abi.h"
+#include "ki_internal.h"
+
+namespace ki {
+
+thread_local std::string g_err;
+
+void fail(int code, const std::string& msg) { throw Error{code, msg}; }
+
+// Growable device buffer.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void ensure(size_t b) {
+    if (b <= bytes) return;
+    release();
+    size_t nb = std::max<size_t>(b, 256);
+    hipError_t e = hipMalloc(&p, nb);
+    if (e != hipSuccess) {
+      p = nullptr;
+      bytes = 0;
+      fail(KI_E_NOMEM, "hipMalloc(" + std::to_string(nb) + ") failed: " +
+                           hipGetErrorString(e));
+    }
+    bytes = nb;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T* as() const {
+    return reinterpret_cast<T*>(p);
+  }
+  ~DevBuf() { release(); }
+};
+
+template <typename T>
+void upload(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
+  b.ensure(sizeof(T) * std::max<size_t>(v.size(), 1));
+  if (!v.empty())
+    KI_HIP(hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(),
+                          hipMemcpyHostToDevice, s));
+}
+
+struct LevelDev {
+  DevBuf fac;       // T / LT / U / X / Acc blocks of the level's boxes
+  DevBuf piv;       // X_rr pivots
+  DevBuf idx;       // skel ++ redundant lists for the sweeps
+  DevBuf sweep;     // SweepDesc array
+  int nsweep = 0, max_n = 0, max_r = 0;
+  int64_t nfac = 0;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t s = nullptr;
+  int dom = 0, sol = 0, pde = 0, npts = 0, ndof = 0;
+  std::vector<double> pts, nrm, w, curv;
+  DevBuf d_pts, d_nrm, d_w, d_curv, d_diag;
+  DevGeom g;
+  bool diag_ready = false;
+  Tree tree;
+  bool tree_built = false;
+  ki_params prm{};
+  bool factored = false;
+  int prec = 64;
+  std::vector<std::unique_ptr<LevelDev>> lev;
+  DevBuf facptrs, pivptrs;          // device arrays of per-level pointers
+  DevBuf diag_sweep;                // all compressed boxes (diag solve)
+  int ndiag = 0, diag_max_r = 0;
+  // top level
+  std::vector<int> allskel;
+  DevBuf top, top_piv, d_allskel, top_vec, top_desc;
+  // scratch
+  DevBuf idm, kbn, qws, qpiv, qrank, idxb, fdesc, qdesc, gdesc, mdesc, sdesc,
+      mapb, tmp;
+  // vectors
+  DevBuf d_b, d_x, d_xt, d_r, d_w2, d_V, d_Z, d_red, d_tmpv, d_dp, d_dout;
+  ki_stats stats{};
+  std::vector<int> boxlog;
+  ~Ctx() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+
+namespace {
+
+double now() {
+  return std::chrono::duration<double>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+ProxySpec make_proxy(const ki_params& p, int dom) {
+  ProxySpec px{};
+  px.npts = p.proxy_points;
+  px.kind = p.proxy_kind;
+  if (dom == 3 && px.kind == 0) {
+    // reference rule: n_theta = 2 n_phi equispaced in theta, GL in phi on
+    // [0, pi] (kernel.h:49-65 with 8 x 4 = 32 points)
+    const int nphi = std::max(1, (int)std::lround(std::sqrt(px.npts / 2.0)));
+    KI_CHECK(nphi <= 64, KI_E_ARG, "too many proxy points for the GL rule");
+    px.nphi = nphi;
+    px.ntheta = 2 * nphi;
+    px.npts = px.nphi * px.ntheta;
+    // Gauss-Legendre nodes by Newton on P_n (ascending), mapped to [0, pi]
+    const int n = nphi;
+    for (int i = 0; i < n; i++) {
+      double x = -std::cos(M_PI * (i + 0.75) / (n + 0.5));
+      double dp = 1;
+      for (int it = 0; it < 100; it++) {
+        double p0 = 1, p1 = x;
+        for (int k = 2; k <= n; k++) {
+          const double p2 = ((2 * k - 1) * x * p1 - (k - 1) * p0) / k;
+          p0 = p1;
+          p1 = p2;
+        }
+        if (n == 1) {
+          p1 = x;
+          p0 = 1;
+        }
+        dp = n * (x * p1 - p0) / (x * x - 1);
+        const double dx = p1 / dp;
+        x -= dx;
+        if (std::fabs(dx) < 1e-16) break;
+      }
+      double p0 = 1, p1 = x;
+      for (int k = 2; k <= n; k++) {
+        const double p2 = ((2 * k - 1) * x * p1 - (k - 1) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      if (n == 1) {
+        p1 = x;
+        p0 = 1;
+      }
+      dp = n * (x * p1 - p0) / (x * x - 1);
+      const double wx = 2.0 / ((1 - x * x) * dp * dp);
+      px.gl_phi[i] = M_PI / 2 * (x + 1);
+      px.gl_w[i] = M_PI / 2 * wx;
+    }
+  }
+  return px;
+}
+
+// Rows of a box's ID matrix (kernel.cpp:432-641): the far/near rule.
+void id_rows(const Ctx& c, const Box& bx, std::vector<int>& rows,
+             bool& special) {
+  const Tree& tr = c.tree;
+  rows.clear();
+  special = bx.level == 1 || (bx.level == 2 && c.dom < 3);
+  if (special) {
+    for (int o : tr.levels[bx.level]) {
+      if (&tr.boxes[o] == &bx) continue;
+      const auto& a = tr.boxes[o].active;
+      rows.insert(rows.end(), a.begin(), a.end());
+    }
+    if (bx.level == 2) {
+      for (int l = 1; l >= 0; l--)
+        for (int o : tr.levels[l])
+          if (tr.boxes[o].leaf) {
+            const auto& a = tr.boxes[o].original;
+            rows.insert(rows.end(), a.begin(), a.end());
+          }
+    }
+    return;
+  }
+  const double lim =
+      std::sqrt((double)c.dom) * c.prm.proxy_ratio * (bx.side / 2);
+  for (int mi : bx.near) {
+    const double* p = c.pts.data() + (int64_t)(mi / c.sol) * c.dom;
+    double dist = 0.;
+    for (int d = 0; d < c.dom; d++) {
+      const double df = bx.center[d] - p[d];
+      dist += df * df;
+    }
+    dist = std::sqrt(dist);
+    if (dist < lim) rows.push_back(mi);
+  }
+}
+
+struct Timer {
+  hipEvent_t a, b;
+  hipStream_t s;
+  explicit Timer(hipStream_t st) : s(st) {
+    KI_HIP(hipEventCreate(&a));
+    KI_HIP(hipEventCreate(&b));
+    KI_HIP(hipEventRecord(a, s));
+  }
+  double stop() {
+    KI_HIP(hipEventRecord(b, s));
+    KI_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    KI_HIP(hipEventElapsedTime(&ms, a, b));
+    return ms * 1e-3;
+  }
+  ~Timer() {
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+  }
+};
+
+template <typename T>
+void skeletonize_t(Ctx& c) {
+  Tree& tr = c.tree;
+  hipStream_t s = c.s;
+  const int L = tr.levels.size();
+  const int d = c.dom, sol = c.sol;
+  for (auto& b : tr.boxes) {
+    b.compressed = false;
+    b.k = b.r = 0;
+    b.skel.clear();
+    b.redundant.clear();
+    b.perm.clear();
+    b.offT = b.offL = b.offU = b.offX = b.offPiv = b.offAcc = -1;
+    b.acc_n = 0;
+    b.fac_level = -1;
+  }
+  c.lev.clear();
+  for (int l = 0; l < L; l++) c.lev.emplace_back(new LevelDev());
+  c.boxlog.clear();
+  ki_stats& st = c.stats;
+  st = ki_stats{};
+  st.levels = L;
+  const ProxySpec px = make_proxy(c.prm, d);
+  const int P = 2 * sol * px.npts;
+  const double t_start = now();
+  std::vector<int> pos(c.ndof, -1);  // DOF -> position scratch
+
+  for (int level = L - 1; level >= 1; level--) {
+    tr.remove_inactive_dofs_at_level(level);
+    // ---- ID matrices -------------------------------------------------
+    std::vector<int> cand, idx;
+    std::vector<FillDesc> fd;
+    std::vector<QrDesc> qd;
+    int64_t idm_el = 0, ws_el = 0, piv_el = 0;
+    int max_cols = 0, max_n = 0;
+    std::vector<int> rows;
+    for (int b : tr.levels[level]) {
+      Box& bx = tr.boxes[b];
+      if (bx.compressed ||
+          (int)bx.active.size() < c.prm.min_dofs_to_compress)
+        continue;
+      bool special = false;
+      id_rows(c, bx, rows, special);
+      const int n = bx.active.size();
+      const int nr = rows.size();
+      const int m = 2 * nr + (special ? 0 : P);
+      if (m == 0) continue;
+      FillDesc f{};
+      f.ncols = n;
+      f.col_idx = idx.size();
+      idx.insert(idx.end(), bx.active.begin(), bx.active.end());
+      const int64_t roff = idx.size();
+      idx.insert(idx.end(), rows.begin(), rows.end());
+      f.nseg = 0;
+      if (nr > 0) {
+        f.seg_kind[f.nseg] = SEG_K;
+        f.seg_rows[f.nseg] = nr;
+        f.seg_idx[f.nseg++] = roff;
+        f.seg_kind[f.nseg] = SEG_KT;
+        f.seg_rows[f.nseg] = nr;
+        f.seg_idx[f.nseg++] = roff;
+      }
+      if (!special) {
+        f.seg_kind[f.nseg] = SEG_PROXY;
+        f.seg_rows[f.nseg] = P;
+        f.seg_idx[f.nseg++] = 0;
+        for (int q = 0; q < 3; q++) f.center[q] = bx.center[q];
+        f.radius = (bx.side / 2) * std::sqrt((double)d) * c.prm.proxy_ratio;
+      }
+      f.out_off = idm_el;
+      f.ld = m;
+      fd.push_back(f);
+      QrDesc q{};
+      q.m = m;
+      q.n = n;
+      q.a_off = idm_el;
+      q.piv_off = piv_el;
+      q.ws_off = ws_el;
+      q.max_rank = n;
+      qd.push_back(q);
+      bx.id_rows = m;
+      idm_el += (int64_t)m * n;
+      piv_el += n;
+      ws_el += 2 * n;
+      max_cols = std::max(max_cols, n);
+      max_n = std::max(max_n, n);
+      cand.push_back(b);
+    }
+    const int nc = cand.size();
+    std::vector<int> rank(nc, 0), hpiv(piv_el);
+    if (nc > 0) {
+      c.idm.ensure(sizeof(T) * idm_el);
+      c.qws.ensure(sizeof(T) * ws_el);
+      c.qpiv.ensure(sizeof(int) * piv_el);
+      c.qrank.ensure(sizeof(int) * nc);
+      upload(c.idxb, idx, s);
+      upload(c.fdesc, fd, s);
+      upload(c.qdesc, qd, s);
+      {
+        Timer tm(s);
+        launch_fill<T>(c.g, px, c.fdesc.as<FillDesc>(), nc, max_cols,
+                       c.idxb.as<int>(), c.idm.as<T>(), s);
+        st.t_fill += tm.stop();
+      }
+      {
+        Timer tm(s);
+        launch_cpqr<T>(c.qdesc.as<QrDesc>(), nc, max_n, c.idm.as<T>(),
+                       c.qpiv.as<int>(), c.qws.as<T>(), c.qrank.as<int>(),
+                       c.prm.id_tol, s);
+        launch_idsolve<T>(c.qdesc.as<QrDesc>(), nc, max_n, c.idm.as<T>(),
+                          c.qrank.as<int>(), s);
+        st.t_qr += tm.stop();
+      }
+      KI_HIP(hipMemcpyAsync(rank.data(), c.qrank.p, sizeof(int) * nc,
+                            hipMemcpyDeviceToHost, s));
+      KI_HIP(hipMemcpyAsync(hpiv.data(), c.qpiv.p, sizeof(int) * piv_el,
+                            hipMemcpyDeviceToHost, s));
+      KI_HIP(hipStreamSynchronize(s));
+      for (int q = 0; q < nc; q++) {
+        const int64_t mn = (int64_t)qd[q].m * qd[q].n;
+        st.bytes_fill += sizeof(T) * (double)mn;
+        const int k = rank[q];
+        st.bytes_qr += sizeof(T) * ((double)mn + (double)k * (qd[q].n - k)) +
+                       4.0 * qd[q].n;
+      }
+    }
+    // ---- ranks -> skeleton / redundant sets ----------------------------
+    std::vector<int> comp;  // compressed boxes at this level
+    std::vector<int> comp_q;
+    for (int q = 0; q < nc; q++) {
+      const int k = rank[q];
+      if (k <= 0) continue;
+      Box& bx = tr.boxes[cand[q]];
+      const int n = bx.active.size();
+      bx.perm.assign(hpiv.begin() + qd[q].piv_off,
+                     hpiv.begin() + qd[q].piv_off + n);
+      bx.skel.clear();
+      bx.redundant.clear();
+      for (int i = 0; i < k; i++) bx.skel.push_back(bx.active[bx.perm[i]]);
+      for (int i = k; i < n; i++) bx.redundant.push_back(bx.active[bx.perm[i]]);
+      bx.k = k;
+      bx.r = n - k;
+      bx.compressed = true;
+      comp.push_back(cand[q]);
+      comp_q.push_back(q);
+      c.boxlog.insert(c.boxlog.end(), {level, n, qd[q].m, k});
+      st.boxes_compressed++;
+    }
+    // ---- factor storage: compressed boxes and pass-through updates -----
+    LevelDev& LD = *c.lev[level];
+    int64_t fac_el = 0, fpiv_el = 0;
+    auto child_has_acc = [&](const Box& bx) {
+      for (int ch : bx.children)
+        if (tr.boxes[ch].acc_n > 0) return true;
+      return false;
+    };
+    std::vector<int> passthru;
+    for (int b : tr.levels[level]) {
+      Box& bx = tr.boxes[b];
+      bx.fac_level = level;
+      if (bx.compressed) {
+        const int64_t k = bx.k, r = bx.r;
+        bx.offT = fac_el;
+        fac_el += k * r;
+        bx.offL = fac_el;
+        fac_el += k * r;
+        bx.offU = fac_el;
+        fac_el += k * r;
+        bx.offX = fac_el;
+        fac_el += r * r;
+        bx.offAcc = fac_el;
+        fac_el += k * k;
+        bx.acc_n = k;
+        bx.offPiv = fpiv_el;
+        fpiv_el += r;
+      } else if (!bx.leaf && child_has_acc(bx) && !bx.active.empty()) {
+        const int64_t n = bx.active.size();
+        bx.offAcc = fac_el;
+        fac_el += n * n;
+        bx.acc_n = n;
+        passthru.push_back(b);
+      }
+    }
+    LD.nfac = fac_el;
+    if (fac_el > 0) {
+      LD.fac.ensure(sizeof(T) * fac_el);
+      launch_zero<T>(LD.fac.as<T>(), fac_el, s);
+    }
+    if (fpiv_el > 0) LD.piv.ensure(sizeof(int) * fpiv_el);
+    T* F = LD.fac.as<T>();
+    const T* Fch = level + 1 < L ? c.lev[level + 1]->fac.as<T>() : nullptr;
+
+    // pass-through updates: Acc = scatter of the children's Acc
+    if (!passthru.empty()) {
+      std::vector<ScatterDesc> sd;
+      std::vector<int> map;
+      int maxn = 0;
+      for (int b : passthru) {
+        Box& bx = tr.boxes[b];
+        for (size_t i = 0; i < bx.active.size(); i++) pos[bx.active[i]] = i;
+        for (int ch : bx.children) {
+          const Box& cb = tr.boxes[ch];
+          if (cb.acc_n == 0) continue;
+          const std::vector<int>& out = cb.compressed ? cb.skel : cb.active;
+          ScatterDesc x{};
+          x.n = cb.acc_n;
+          x.src_off = cb.offAcc;
+          x.map_off = map.size();
+          for (int dof : out) {
+            KI_CHECK(pos[dof] >= 0, KI_E_STATE, "update DOF outside parent");
+            map.push_back(pos[dof]);
+          }
+          x.dst_off = bx.offAcc;
+          x.ld = bx.acc_n;
+          x.acc_off = -1;
+          sd.push_back(x);
+          maxn = std::max(maxn, x.n);
+        }
+        for (int dof : bx.active) pos[dof] = -1;
+      }
+      upload(c.sdesc, sd, s);
+      upload(c.mapb, map, s);
+      launch_scatter<T, T>(c.sdesc.as<ScatterDesc>(), sd.size(), maxn, Fch,
+                           c.mapb.as<int>(), F, (T)1, (T*)nullptr, s);
+      KI_HIP(hipStreamSynchronize(s));
+    }
+    if (comp.empty()) continue;
+
+    // ---- decouple (skel_factorization.cpp:91-127) ----------------------
+    Timer tdec(s);
+    const int ncp = comp.size();
+    std::vector<int> bidx;
+    std::vector<FillDesc> kf;
+    std::vector<int64_t> kb_off(ncp);
+    int64_t kbn_el = 0;
+    int maxn = 0, maxk = 0, maxr = 0;
+    for (int q = 0; q < ncp; q++) {
+      Box& bx = tr.boxes[comp[q]];
+      const int n = bx.k + bx.r;
+      FillDesc f{};
+      f.ncols = n;
+      f.col_idx = bidx.size();
+      bidx.insert(bidx.end(), bx.skel.begin(), bx.skel.end());
+      bidx.insert(bidx.end(), bx.redundant.begin(), bx.redundant.end());
+      f.nseg = 1;
+      f.seg_kind[0] = SEG_K;
+      f.seg_rows[0] = n;
+      f.seg_idx[0] = f.col_idx;
+      f.out_off = kbn_el;
+      f.ld = n;
+      kb_off[q] = kbn_el;
+      kbn_el += (int64_t)n * n;
+      kf.push_back(f);
+      maxn = std::max(maxn, n);
+      maxk = std::max(maxk, bx.k);
+      maxr = std::max(maxr, bx.r);
+    }
+    c.kbn.ensure(sizeof(T) * kbn_el);
+    T* K = c.kbn.as<T>();
+    upload(c.idxb, bidx, s);
+    upload(c.fdesc, kf, s);
+    launch_fill<T>(c.g, px, c.fdesc.as<FillDesc>(), ncp, maxn,
+                   c.idxb.as<int>(), K, s);
+    // subtract children's accumulated updates; seed Acc with their skel part
+    {
+      std::vector<ScatterDesc> sd;
+      std::vector<int> map;
+      int mxn = 0;
+      for (int q = 0; q < ncp; q++) {
+        Box& bx = tr.boxes[comp[q]];
+        const int n = bx.k + bx.r;
+        for (int i = 0; i < bx.k; i++) pos[bx.skel[i]] = i;
+        for (int i = 0; i < bx.r; i++) pos[bx.redundant[i]] = bx.k + i;
+        for (int ch : bx.children) {
+          const Box& cb = tr.boxes[ch];
+          if (cb.acc_n == 0) continue;
+          const std::vector<int>& out = cb.compressed ? cb.skel : cb.active;
+          ScatterDesc x{};
+          x.n = cb.acc_n;
+          x.src_off = cb.offAcc;
+          x.map_off = map.size();
+          for (int dof : out) {
+            KI_CHECK(pos[dof] >= 0, KI_E_STATE, "update DOF outside box");
+            map.push_back(pos[dof]);
+          }
+          x.dst_off = kb_off[q];
+          x.ld = n;
+          x.acc_off = bx.offAcc;
+          x.acc_ld = bx.k;
+          x.acc_k = bx.k;
+          sd.push_back(x);
+          mxn = std::max(mxn, x.n);
+        }
+        for (int dof : bx.skel) pos[dof] = -1;
+        for (int dof : bx.redundant) pos[dof] = -1;
+      }
+      if (!sd.empty()) {
+        upload(c.sdesc, sd, s);
+        upload(c.mapb, map, s);
+        launch_scatter<T, T>(c.sdesc.as<ScatterDesc>(), sd.size(), mxn, Fch,
+                             c.mapb.as<int>(), K, (T)-1, F, s);
+      }
+    }
+    // copy T out of the ID matrices
+    {
+      std::vector<MatDesc> md(ncp);
+      int mm = 0, nn = 0;
+      for (int q = 0; q < ncp; q++) {
+        const Box& bx = tr.boxes[comp[q]];
+        const QrDesc& qq = qd[comp_q[q]];
+        MatDesc& x = md[q];
+        x = MatDesc{};
+        x.m = bx.k;
+        x.n = bx.r;
+        x.off = qq.a_off + (int64_t)qq.m * bx.k;
+        x.ld = qq.m;
+        x.dst_off = bx.offT;
+        x.ldd = bx.k;
+        mm = std::max(mm, x.m);
+        nn = std::max(nn, x.n);
+      }
+      upload(c.mdesc, md, s);
+      launch_copy<T>(c.mdesc.as<MatDesc>(), ncp, mm, nn, c.idm.as<T>(), F, 0,
+                     s);
+    }
+    // four GEMMs
+    auto gemm_batch = [&](auto make, const T* A, const T* B, T* C) {
+      std::vector<GemmDesc> gd(ncp);
+      int mm = 0, nn = 0;
+      for (int q = 0; q < ncp; q++) {
+        gd[q] = make(tr.boxes[comp[q]], kb_off[q]);
+        mm = std::max(mm, gd[q].m);
+        nn = std::max(nn, gd[q].n);
+        st.flops_schur += 2.0 * gd[q].m * gd[q].n * gd[q].k;
+      }
+      upload(c.gdesc, gd, s);
+      launch_gemm<T>(c.gdesc.as<GemmDesc>(), ncp, mm, nn, A, B, C, s);
+      KI_HIP(hipStreamSynchronize(s));
+    };
+    // (a) K_rs' = K_rs - T^T K_ss
+    gemm_batch(
+        [&](const Box& bx, int64_t kb) {
+          const int n = bx.k + bx.r;
+          return GemmDesc{bx.r, bx.k, bx.k, bx.offT, kb, kb + bx.k, bx.k, n,
+                          n,    1,    0,    -1.0,    1.0};
+        },
+        F, K, K);
+    // (c1) X = K_rr - T^T K_sr (original K_sr)
+    gemm_batch(
+        [&](const Box& bx, int64_t kb) {
+          const int n = bx.k + bx.r;
+          return GemmDesc{bx.r, bx.r, bx.k, bx.offT, kb + (int64_t)n * bx.k,
+                          kb + bx.k + (int64_t)n * bx.k, bx.k, n, n, 1, 0,
+                          -1.0, 1.0};
+        },
+        F, K, K);
+    // (c2) X -= K_rs' T
+    gemm_batch(
+        [&](const Box& bx, int64_t kb) {
+          const int n = bx.k + bx.r;
+          return GemmDesc{bx.r, bx.r, bx.k, kb + bx.k, bx.offT,
+                          kb + bx.k + (int64_t)n * bx.k, n, bx.k, n, 0, 0,
+                          -1.0, 1.0};
+        },
+        K, F, K);
+    // (b) K_sr' = K_sr - K_ss T
+    gemm_batch(
+        [&](const Box& bx, int64_t kb) {
+          const int n = bx.k + bx.r;
+          return GemmDesc{bx.k, bx.r, bx.k, kb, bx.offT,
+                          kb + (int64_t)n * bx.k, n, bx.k, n, 0, 0, -1.0, 1.0};
+        },
+        K, F, K);
+    // copies: X -> X slot, K_rs' -> U slot, K_sr'^T -> L^T slot
+    {
+      std::vector<MatDesc> mx(ncp), mu(ncp), ml(ncp);
+      int mr = 0, mk = 0;
+      for (int q = 0; q < ncp; q++) {
+        const Box& bx = tr.boxes[comp[q]];
+        const int n = bx.k + bx.r;
+        const int64_t kb = kb_off[q];
+        mx[q] = MatDesc{};
+        mx[q].m = bx.r;
+        mx[q].n = bx.r;
+        mx[q].off = kb + bx.k + (int64_t)n * bx.k;
+        mx[q].ld = n;
+        mx[q].dst_off = bx.offX;
+        mx[q].ldd = bx.r;
+        mu[q] = MatDesc{};
+        mu[q].m = bx.r;
+        mu[q].n = bx.k;
+        mu[q].off = kb + bx.k;
+        mu[q].ld = n;
+        mu[q].dst_off = bx.offU;
+        mu[q].ldd = bx.r;
+        ml[q] = MatDesc{};
+        ml[q].m = bx.k;
+        ml[q].n = bx.r;
+        ml[q].off = kb + (int64_t)n * bx.k;
+        ml[q].ld = n;
+        ml[q].dst_off = bx.offL;
+        ml[q].ldd = bx.r;
+        mr = std::max(mr, bx.r);
+        mk = std::max(mk, bx.k);
+      }
+      upload(c.mdesc, mx, s);
+      launch_copy<T>(c.mdesc.as<MatDesc>(), ncp, mr, mr, K, F, 0, s);
+      KI_HIP(hipStreamSynchronize(s));
+      upload(c.mdesc, mu, s);
+      launch_copy<T>(c.mdesc.as<MatDesc>(), ncp, mr, mk, K, F, 0, s);
+      KI_HIP(hipStreamSynchronize(s));
+      upload(c.mdesc, ml, s);
+      launch_copy<T>(c.mdesc.as<MatDesc>(), ncp, mk, mr, K, F, 1, s);
+      KI_HIP(hipStreamSynchronize(s));
+    }
+    // LU(X_rr), U = X^{-1} K_rs', L^T = X^{-T} K_sr'^T
+    {
+      std::vector<MatDesc> ml(ncp), mU(ncp), mL(ncp);
+      for (int q = 0; q < ncp; q++) {
+        const Box& bx = tr.boxes[comp[q]];
+        MatDesc x{};
+        x.m = x.n = bx.r;
+        x.off = bx.offX;
+        x.ld = bx.r;
+        x.piv_off = bx.offPiv;
+        ml[q] = x;
+        x.nrhs = bx.k;
+        x.ldb = bx.r;
+        x.b_off = bx.offU;
+        mU[q] = x;
+        x.b_off = bx.offL;
+        mL[q] = x;
+        const double r = bx.r, k = bx.k;
+        st.flops_schur += 2.0 / 3.0 * r * r * r + 4.0 * r * r * k;
+      }
+      upload(c.mdesc, ml, s);
+      batched_lu<T>(ml, c.mdesc.as<MatDesc>(), F, LD.piv.as<int>(), s);
+      upload(c.mdesc, mU, s);
+      batched_getrs<T>(mU, c.mdesc.as<MatDesc>(), F, LD.piv.as<int>(), F,
+                       false, s);
+      upload(c.mdesc, mL, s);
+      batched_getrs<T>(mL, c.mdesc.as<MatDesc>(), F, LD.piv.as<int>(), F,
+                       true, s);
+    }
+    // Schur complement: Acc += L K_rs' = (L^T)^T K_rs'
+    gemm_batch(
+        [&](const Box& bx, int64_t kb) {
+          const int n = bx.k + bx.r;
+          return GemmDesc{bx.k, bx.k, bx.r, bx.offL, kb + bx.k, bx.offAcc,
+                          bx.r, n,    bx.k, 1,       0,         1.0,
+                          1.0};
+        },
+        F, K, F);
+    st.t_decouple += tdec.stop();
+  }
+
+  // ---- top level (skel_factorization.cpp:285-310), FP64 -----------------
+  tr.remove_inactive_dofs_at_all_boxes();
+  c.allskel = tr.boxes[tr.root].active;
+  const int nt = c.allskel.size();
+  st.top_size = nt;
+  if (nt > 0) {
+    Timer ttop(s);
+    c.top.ensure(sizeof(double) * (int64_t)nt * nt);
+    c.top_piv.ensure(sizeof(int) * nt);
+    upload(c.d_allskel, c.allskel, s);
+    FillDesc f{};
+    f.ncols = nt;
+    f.nseg = 1;
+    f.seg_kind[0] = SEG_K;
+    f.seg_rows[0] = nt;
+    f.seg_idx[0] = 0;
+    f.col_idx = 0;
+    f.out_off = 0;
+    f.ld = nt;
+    std::vector<FillDesc> fv{f};
+    upload(c.fdesc, fv, s);
+    launch_fill<double>(c.g, px, c.fdesc.as<FillDesc>(), 1, nt,
+                        c.d_allskel.as<int>(), c.top.as<double>(), s);
+    if (L > 1) {
+      std::vector<ScatterDesc> sd;
+      std::vector<int> map;
+      int mxn = 0;
+      for (int i = 0; i < nt; i++) pos[c.allskel[i]] = i;
+      for (int ch : tr.boxes[tr.root].children) {
+        const Box& cb = tr.boxes[ch];
+        if (cb.acc_n == 0) continue;
+        const std::vector<int>& out = cb.compressed ? cb.skel : cb.active;
+        ScatterDesc x{};
+        x.n = cb.acc_n;
+        x.src_off = cb.offAcc;
+        x.map_off = map.size();
+        for (int dof : out) {
+          KI_CHECK(pos[dof] >= 0, KI_E_STATE, "update DOF outside allskel");
+          map.push_back(pos[dof]);
+        }
+        x.dst_off = 0;
+        x.ld = nt;
+        x.acc_off = -1;
+        sd.push_back(x);
+        mxn = std::max(mxn, x.n);
+      }
+      for (int i = 0; i < nt; i++) pos[c.allskel[i]] = -1;
+      if (!sd.empty()) {
+        upload(c.sdesc, sd, s);
+        upload(c.mapb, map, s);
+        launch_scatter<T, double>(c.sdesc.as<ScatterDesc>(), sd.size(), mxn,
+                                  c.lev[1]->fac.as<T>(), c.mapb.as<int>(),
+                                  c.top.as<double>(), -1.0, (double*)nullptr,
+                                  s);
+      }
+    }
+    std::vector<MatDesc> md(1);
+    md[0] = MatDesc{};
+    md[0].m = md[0].n = nt;
+    md[0].off = 0;
+    md[0].ld = nt;
+    md[0].piv_off = 0;
+    md[0].nrhs = 1;
+    md[0].b_off = 0;
+    md[0].ldb = nt;
+    upload(c.top_desc, md, s);
+    batched_lu<double>(md, c.top_desc.as<MatDesc>(), c.top.as<double>(),
+                       c.top_piv.as<int>(), s);
+    st.flops_top = 2.0 / 3.0 * (double)nt * nt * nt;
+    st.t_top = ttop.stop();
+  }
+
+  // ---- solve plans -------------------------------------------------------
+  std::vector<T*> fptr(L, nullptr);
+  std::vector<int*> pptr(L, nullptr);
+  std::vector<SweepDesc> all;
+  int64_t dev_bytes = 0;
+  for (int l = 0; l < L; l++) {
+    LevelDev& LD = *c.lev[l];
+    fptr[l] = LD.fac.as<T>();
+    pptr[l] = LD.piv.as<int>();
+    dev_bytes += LD.fac.bytes + LD.piv.bytes;
+    std::vector<SweepDesc> sw;
+    std::vector<int> li;
+    LD.max_n = LD.max_r = 0;
+    for (int b : tr.levels[l]) {
+      const Box& bx = tr.boxes[b];
+      if (!bx.compressed) continue;
+      SweepDesc x{};
+      x.k = bx.k;
+      x.r = bx.r;
+      x.s_idx = li.size();
+      li.insert(li.end(), bx.skel.begin(), bx.skel.end());
+      x.r_idx = li.size();
+      li.insert(li.end(), bx.redundant.begin(), bx.redundant.end());
+      x.offT = bx.offT;
+      x.offL = bx.offL;
+      x.offU = bx.offU;
+      x.offX = bx.offX;
+      x.offPiv = bx.offPiv;
+      x.buf = l;
+      sw.push_back(x);
+      LD.max_n = std::max(LD.max_n, bx.k + bx.r);
+      LD.max_r = std::max(LD.max_r, bx.r);
+    }
+    LD.nsweep = sw.size();
+    KI_CHECK((int64_t)LD.max_n * (int64_t)sizeof(T) <= 64 * 1024, KI_E_ARG,
+             "box too large for the LDS-staged sweep kernel");
+    if (!sw.empty()) {
+      upload(LD.idx, li, s);
+      upload(LD.sweep, sw, s);
+    }
+  }
+  upload(c.facptrs, fptr, s);
+  upload(c.pivptrs, pptr, s);
+  KI_HIP(hipStreamSynchronize(s));
+  st.device_bytes = dev_bytes + c.top.bytes;
+  st.t_total = now() - t_start;
+}
+
+// x_out = A^{-1} b_in with the factorization (device FP64 vectors).
+template <typename T>
+void solve_t(Ctx& c, const double* b_in, double* x_out) {
+  hipStream_t s = c.s;
+  Tree& tr = c.tree;
+  const int L = tr.levels.size();
+  c.d_xt.ensure(sizeof(T) * c.ndof);
+  T* xt = c.d_xt.as<T>();
+  launch_convert<T, double>(b_in, xt, c.ndof, s);
+  T* const* fb = c.facptrs.as<T*>();
+  int* const* pb = c.pivptrs.as<int*>();
+  for (int l = L - 1; l >= 0; l--) {
+    LevelDev& LD = *c.lev[l];
+    if (LD.nsweep == 0) continue;
+    launch_sweep<T>(LD.sweep.as<SweepDesc>(), LD.nsweep, LD.idx.as<int>(), fb,
+                    xt, 1, s, LD.max_n);
+  }
+  for (int l = L - 1; l >= 0; l--) {
+    LevelDev& LD = *c.lev[l];
+    if (LD.nsweep == 0) continue;
+    launch_diag_solve<T>(LD.sweep.as<SweepDesc>(), LD.nsweep, LD.idx.as<int>(),
+                         fb, pb, xt, s, std::max(LD.max_r, 1));
+  }
+  const int nt = c.allskel.size();
+  if (nt > 0) {
+    c.top_vec.ensure(sizeof(double) * nt);
+    double* tv = c.top_vec.as<double>();
+    launch_gather<T, double>(xt, c.d_allskel.as<int>(), nt, tv, s);
+    std::vector<MatDesc> md(1);
+    md[0] = MatDesc{};
+    md[0].m = md[0].n = nt;
+    md[0].ld = nt;
+    md[0].nrhs = 1;
+    md[0].ldb = nt;
+    batched_getrs<double>(md, c.top_desc.as<MatDesc>(), c.top.as<double>(),
+                          c.top_piv.as<int>(), tv, false, s);
+    launch_scatter_vec<double, T>(tv, c.d_allskel.as<int>(), nt, xt, s);
+  }
+  for (int l = 0; l < L; l++) {
+    LevelDev& LD = *c.lev[l];
+    if (LD.nsweep == 0) continue;
+    launch_sweep<T>(LD.sweep.as<SweepDesc>(), LD.nsweep, LD.idx.as<int>(), fb,
+                    xt, 0, s, LD.max_n);
+  }
+  launch_convert<double, T>(xt, x_out, c.ndof, s);
+}
+
+void solve_dev(Ctx& c, const double* b, double* x) {
+  KI_CHECK(c.factored, KI_E_STATE, "solve called before skeletonize");
+  if (c.prec == 32)
+    solve_t<float>(c, b, x);
+  else
+    solve_t<double>(c, b, x);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 vector helpers for GMRES.
+// ---------------------------------------------------------------------------
+__global__ void dot_partial(const double* a, const double* b, int n,
+                            double* out) {
+  __shared__ double red[256];
+  double s = 0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+    s += a[i] * b[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+__global__ void axpby_kernel(double a, const double* x, double b, double* y,
+                             int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) y[i] = a * x[i] + b * y[i];
+}
+
+double dot(Ctx& c, const double* a, const double* b, int n) {
+  const int nb = std::min(256, (n + 255) / 256);
+  c.d_red.ensure(sizeof(double) * 256);
+  hipLaunchKernelGGL(dot_partial, dim3(nb), dim3(256), 0, c.s, a, b, n,
+                     c.d_red.as<double>());
+  double h[256];
+  KI_HIP(hipMemcpyAsync(h, c.d_red.p, sizeof(double) * nb,
+                        hipMemcpyDeviceToHost, c.s));
+  KI_HIP(hipStreamSynchronize(c.s));
+  double s = 0;
+  for (int i = 0; i < nb; i++) s += h[i];
+  return s;
+}
+
+// y = a x + b y
+void axpby(Ctx& c, double a, const double* x, double b, double* y, int n) {
+  hipLaunchKernelGGL(axpby_kernel, dim3((n + 255) / 256), dim3(256), 0, c.s, a,
+                     x, b, y, n);
+  KI_HIP(hipGetLastError());
+}
+
+void matvec_dev(Ctx& c, const double* mu, double* y) {
+  launch_matvec(c.g, mu, y, c.s);
+}
+
+// Right-preconditioned restarted GMRES (modified Gram-Schmidt, Givens),
+// device vectors b, x (x is the initial guess on entry).
+void gmres_dev(Ctx& c, const double* b, double* x, double tol, int maxit,
+               int restart, int* iters, double* relres) {
+  const int n = c.ndof;
+  restart = std::max(1, restart);
+  c.d_r.ensure(sizeof(double) * n);
+  c.d_w2.ensure(sizeof(double) * n);
+  c.d_V.ensure(sizeof(double) * (size_t)n * (restart + 1));
+  c.d_Z.ensure(sizeof(double) * (size_t)n * restart);
+  double* r = c.d_r.as<double>();
+  double* w = c.d_w2.as<double>();
+  double* V = c.d_V.as<double>();
+  double* Z = c.d_Z.as<double>();
+  const double bn = std::sqrt(dot(c, b, b, n));
+  int it = 0;
+  double rel = 0;
+  if (bn == 0) {
+    launch_zero<double>(x, n, c.s);
+    if (iters) *iters = 0;
+    if (relres) *relres = 0;
+    return;
+  }
+  std::vector<double> H((size_t)(restart + 1) * restart), cs(restart),
+      sn(restart), gv(restart + 1), y(restart);
+  for (;;) {
+    matvec_dev(c, x, r);
+    axpby(c, 1.0, b, -1.0, r, n);  // r = b - A x
+    const double beta = std::sqrt(dot(c, r, r, n));
+    rel = beta / bn;
+    if (rel <= tol || it >= maxit) break;
+    KI_HIP(hipMemsetAsync(V, 0, sizeof(double) * n, c.s));
+    axpby(c, 1.0 / beta, r, 0.0, V, n);
+    std::fill(gv.begin(), gv.end(), 0.0);
+    gv[0] = beta;
+    int j = 0;
+    for (; j < restart && it < maxit; j++, it++) {
+      double* vj = V + (size_t)n * j;
+      double* zj = Z + (size_t)n * j;
+      solve_dev(c, vj, zj);
+      matvec_dev(c, zj, w);
+      for (int i = 0; i <= j; i++) {
+        double* vi = V + (size_t)n * i;
+        const double h = dot(c, w, vi, n);
+        H[i + (size_t)(restart + 1) * j] = h;
+        axpby(c, -h, vi, 1.0, w, n);
+      }
+      const double hn = std::sqrt(dot(c, w, w, n));
+      H[j + 1 + (size_t)(restart + 1) * j] = hn;
+      double* vn = V + (size_t)n * (j + 1);
+      KI_HIP(hipMemsetAsync(vn, 0, sizeof(double) * n, c.s));
+      if (hn > 0) axpby(c, 1.0 / hn, w, 0.0, vn, n);
+      for (int i = 0; i < j; i++) {  // apply previous rotations
+        double& a = H[i + (size_t)(restart + 1) * j];
+        double& bb = H[i + 1 + (size_t)(restart + 1) * j];
+        const double t = cs[i] * a + sn[i] * bb;
+        bb = -sn[i] * a + cs[i] * bb;
+        a = t;
+      }
+      double& a = H[j + (size_t)(restart + 1) * j];
+      double& bb = H[j + 1 + (size_t)(restart + 1) * j];
+      const double den = std::hypot(a, bb);
+      cs[j] = den == 0 ? 1 : a / den;
+      sn[j] = den == 0 ? 0 : bb / den;
+      a = den;
+      bb = 0;
+      gv[j + 1] = -sn[j] * gv[j];
+      gv[j] = cs[j] * gv[j];
+      if (std::fabs(gv[j + 1]) / bn <= tol * 0.5) {
+        j++;
+        it++;
+        break;
+      }
+    }
+    // y = H^{-1} g, x += Z y
+    for (int i = j - 1; i >= 0; i--) {
+      double sum = gv[i];
+      for (int q = i + 1; q < j; q++)
+        sum -= H[i + (size_t)(restart + 1) * q] * y[q];
+      y[i] = sum / H[i + (size_t)(restart + 1) * i];
+    }
+    for (int i = 0; i < j; i++) axpby(c, y[i], Z + (size_t)n * i, 1.0, x, n);
+  }
+  KI_HIP(hipStreamSynchronize(c.s));
+  if (iters) *iters = it;
+  if (relres) *relres = rel;
+}
+
+}  // namespace
+}  // namespace ki
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+using ki::Ctx;
+
+struct ki_ctx {
+  Ctx c;
+};
+
+template <typename F>
+static int guard(F&& f) {
+  try {
+    f();
+    ki::g_err.clear();
+    return KI_OK;
+  } catch (const ki::Error& e) {
+    ki::g_err = e.msg;
+    return e.code == 3 ? KI_E_DEVICE : (e.code < 0 ? e.code : KI_E_ARG);
+  } catch (const std::exception& e) {
+    ki::g_err = e.what();
+    return KI_E_STATE;
+  }
+}
+
+extern "C" {
+
+void ki_params_default(ki_params* p) {
+  std::memset(p, 0, sizeof(*p));
+  p->id_tol = 1e-6;
+  p->min_dofs_to_compress = 64;
+  p->proxy_points = 128;
+  p->proxy_ratio = 2.0;
+  p->proxy_kind = 0;
+  p->precision = 64;
+  p->qr_full = 0;
+  p->use_predictor = 0;
+}
+
+const char* ki_last_error(void) { return ki::g_err.c_str(); }
+
+int ki_ctx_create(int device, ki_ctx** out) {
+  return guard([&] {
+    KI_CHECK(out != nullptr, KI_E_ARG, "null output pointer");
+    int nd = 0;
+    KI_HIP(hipGetDeviceCount(&nd));
+    KI_CHECK(device >= 0 && device < nd, KI_E_ARG, "no such HIP device");
+    KI_HIP(hipSetDevice(device));
+    auto* k = new ki_ctx();
+    k->c.device = device;
+    KI_HIP(hipStreamCreate(&k->c.s));
+    *out = k;
+  });
+}
+
+void ki_ctx_destroy(ki_ctx* ctx) { delete ctx; }
+
+int ki_set_geometry(ki_ctx* ctx, int dom, int pde, const double* pts,
+                    const double* nrm, const double* w, const double* curv,
+                    int npts) {
+  return guard([&] {
+    KI_CHECK(ctx && pts && nrm && w && npts > 0, KI_E_ARG, "bad geometry");
+    KI_CHECK(dom == 2 || dom == 3, KI_E_ARG, "dom must be 2 or 3");
+    KI_CHECK(pde == KI_LAPLACE || pde == KI_STOKES, KI_E_ARG,
+             "pde must be KI_LAPLACE or KI_STOKES");
+    KI_CHECK(dom == 3 || curv != nullptr, KI_E_ARG,
+             "2D geometry needs curvatures");
+    Ctx& c = ctx->c;
+    KI_HIP(hipSetDevice(c.device));
+    c.dom = dom;
+    c.pde = pde;
+    c.sol = pde == KI_LAPLACE ? 1 : dom;
+    c.npts = npts;
+    c.ndof = npts * c.sol;
+    c.pts.assign(pts, pts + (int64_t)npts * dom);
+    c.nrm.assign(nrm, nrm + (int64_t)npts * dom);
+    c.w.assign(w, w + npts);
+    if (curv)
+      c.curv.assign(curv, curv + npts);
+    else
+      c.curv.assign(npts, 0.0);
+    upload(c.d_pts, c.pts, c.s);
+    upload(c.d_nrm, c.nrm, c.s);
+    upload(c.d_w, c.w, c.s);
+    upload(c.d_curv, c.curv, c.s);
+    c.g.npts = npts;
+    c.g.dom = dom;
+    c.g.sol = c.sol;
+    c.g.pde = pde;
+    c.g.pts = c.d_pts.as<double>();
+    c.g.nrm = c.d_nrm.as<double>();
+    c.g.w = c.d_w.as<double>();
+    c.g.curv = c.d_curv.as<double>();
+    c.g.diag = nullptr;
+    c.diag_ready = false;
+    c.tree_built = false;
+    c.factored = false;
+    KI_HIP(hipStreamSynchronize(c.s));
+  });
+}
+
+int ki_compute_diag3d(ki_ctx* ctx) {
+  return guard([&] {
+    Ctx& c = ctx->c;
+    KI_CHECK(c.dom == 3, KI_E_STATE, "diagonals are computed for 3D only");
+    KI_HIP(hipSetDevice(c.device));
+    c.d_diag.ensure(sizeof(double) * (size_t)c.npts * (c.sol == 1 ? 1 : 9));
+    c.g.diag = c.d_diag.as<double>();
+    ki::launch_diag3d(c.g, c.s);
+    KI_HIP(hipStreamSynchronize(c.s));
+    c.diag_ready = true;
+  });
+}
+
+int ki_get_diag3d(ki_ctx* ctx, double* out, int64_t count) {
+  return guard([&] {
+    Ctx& c = ctx->c;
+    KI_CHECK(c.diag_ready, KI_E_STATE, "diagonals not computed");
+    const int64_t n = (int64_t)c.npts * (c.sol == 1 ? 1 : 9);
+    KI_CHECK(count >= n, KI_E_ARG, "output too small");
+    KI_HIP(hipMemcpy(out, c.d_diag.p, sizeof(double) * n,
+                     hipMemcpyDeviceToHost));
+  });
+}
+
+int ki_build_tree(ki_ctx* ctx, int max_leaf_dofs) {
+  return guard([&] {
+    Ctx& c = ctx->c;
+    KI_CHECK(c.npts > 0, KI_E_STATE, "set geometry first");
+    KI_CHECK(max_leaf_dofs >= 1, KI_E_ARG, "max_leaf_dofs must be >= 1");
+    c.tree.build(c.pts.data(), c.npts, c.dom, c.sol, max_leaf_dofs);
+    c.tree_built = true;
+    c.factored = false;
+  });
+}
+
+int ki_tree_num_levels(ki_ctx* ctx) {
+  return ctx->c.tree_built ? (int)ctx->c.tree.levels.size() : 0;
+}
+
+int ki_tree_level_size(ki_ctx* ctx, int level) {
+  const auto& t = ctx->c.tree;
+  if (!ctx->c.tree_built || level < 0 || level >= (int)t.levels.size())
+    return KI_E_ARG;
+  return t.levels[level].size();
+}
+
+static const ki::Box* get_box(Ctx& c, int level, int idx) {
+  KI_CHECK(c.tree_built, KI_E_STATE, "no tree");
+  KI_CHECK(level >= 0 && level < (int)c.tree.levels.size(), KI_E_ARG,
+           "bad level");
+  KI_CHECK(idx >= 0 && idx < (int)c.tree.levels[level].size(), KI_E_ARG,
+           "bad box index");
+  return &c.tree.boxes[c.tree.levels[level][idx]];
+}
+
+int ki_tree_box(ki_ctx* ctx, int level, int idx, int* info, double* geom) {
+  return guard([&] {
+    Ctx& c = ctx->c;
+    const ki::Box* b = get_box(c, level, idx);
+    if (info) {
+      info[0] = b->leaf;
+      info[1] = b->original.size();
+      info[2] = b->active.size();
+      info[3] = b->neighbors.size();
+      info[4] = b->compressed;
+      info[5] = b->k;
+      info[6] = b->near.size();
+      info[7] = b->parent >= 0 ? c.tree.boxes[b->parent].index_in_level : -1;
+    }
+    if (geom) {
+      for (int q = 0; q < 3; q++) geom[q] = b->center[q];
+      geom[3] = b->side;
+    }
+  });
+}
+
+int ki_tree_box_list(ki_ctx* ctx, int level, int idx, int which, int* out,
+                     int cap, int* count) {
+  return guard([&] {
+    Ctx& c = ctx->c;
+    const ki::Box* b = get_box(c, level, idx);
+    std::vector<int> tmp;
+    const std::vector<int>* v = nullptr;
+    switch (which) {
+      case 0: v = &b->original; break;
+      case 1: v = &b->active; break;
+      case 2: v = &b->skel; break;
+      case 3: v = &b->redundant; break;
+      case 4: v = &b->near; break;
+      case 5:
+        for (int nb : b->neighbors)
+          tmp.push_back((c.tree.boxes[nb].level << 20) +
+                        c.tree.boxes[nb].index_in_level);
+        v = &tmp;
+        break;
+      default: ki::fail(KI_E_ARG, "bad list selector");
+    }
+    if (count) *count = v->size();
+    const int nc = std::min<int>(cap, v->size());
+    if (out && nc > 0) std::memcpy(out, v->data(), sizeof(int) * nc);
+  });
+}
+
+int ki_skeletonize(ki_ctx* ctx, const ki_params* params) {
+  return guard([&] {
+    Ctx& c = ctx->c;
+    KI_CHECK(c.tree_built, KI_E_STATE, "build the tree first");
+    KI_CHECK(c.dom == 2 || c.diag_ready, KI_E_STATE,
+             "3D kernels need ki_compute_diag3d first");
+    KI_HIP(hipSetDevice(c.device));
+    if (params)
+      c.prm = *params;
+    else
+      ki_params_default(&c.prm);
+    KI_CHECK(c.prm.precision == 32 || c.prm.precision == 64, KI_E_ARG,
+             "precision must be 32 or 64");
+    KI_CHECK(c.prm.id_tol > 0, KI_E_ARG, "id_tol must be positive");
+    KI_CHECK(c.prm.proxy_points > 0, KI_E_ARG, "proxy_points must be > 0");
+    c.factored = false;
+    c.prec = c.prm.precision;
+    if (c.prec == 32)
+      ki::skeletonize_t<float>(c);
+    else
+      ki::skeletonize_t<double>(c);
+    c.factored = true;
+  });
+}
+
+int ki_solve(ki_ctx* ctx, const double* b, double* x) {
+  return guard([&] {
+    Ctx& c = ctx->c;
+    KI_HIP(hipSetDevice(c.device));
+    c.d_b.ensure(sizeof(double) * c.ndof);
+    c.d_x.ensure(sizeof(double) * c.ndof);
+    KI_HIP(hipMemcpyAsync(c.d_b.p, b, sizeof(double) * c.ndof,
+                          hipMemcpyHostToDevice, c.s));
+    ki::Timer tm(c.s);
+    ki::solve_dev(c, c.d_b.as<double>(), c.d_x.as<double>());
+    c.stats.t_solve_last = tm.stop();
+    KI_HIP(hipMemcpyAsync(x, c.d_x.p, sizeof(double) * c.ndof,
+                          hipMemcpyDeviceToHost, c.s));
+    KI_HIP(hipStreamSynchronize(c.s));
+  });
+}
+
+int ki_matvec(ki_ctx* ctx, const double* mu, double* y) {
+  return guard([&] {
+    Ctx& c = ctx->c;
+    KI_CHECK(c.npts > 0, KI_E_STATE, "no geometry");
+    KI_CHECK(c.dom == 2 || c.diag_ready, KI_E_STATE,
+             "3D kernels need ki_compute_diag3d first");
+    KI_HIP(hipSetDevice(c.device));
+    c.d_tmpv.ensure(sizeof(double) * c.ndof);
+    c.d_w2.ensure(sizeof(double) * c.ndof);
+    KI_HIP(hipMemcpyAsync(c.d_tmpv.p, mu, sizeof(double) * c.ndof,
+                          hipMemcpyHostToDevice, c.s));
+    ki::Timer tm(c.s);
+    ki::matvec_dev(c, c.d_tmpv.as<double>(), c.d_w2.as<double>());
+    c.stats.t_matvec_last = tm.stop();
+    KI_HIP(hipMemcpyAsync(y, c.d_w2.p, sizeof(double) * c.ndof,
+                          hipMemcpyDeviceToHost, c.s));
+    KI_HIP(hipStreamSynchronize(c.s));
+  });
+}
+
+int ki_set_rhs(ki_ctx* ctx, const double* b) {
+  return guard([&] {
+    Ctx& c = ctx->c;
+    KI_CHECK(c.ndof > 0, KI_E_STATE, "no geometry");
+    c.d_b.ensure(sizeof(double) * c.ndof);
+    KI_HIP(hipMemcpyAsync(c.d_b.p, b, sizeof(double) * c.ndof,
+                          hipMemcpyHostToDevice, c.s));
+    KI_HIP(hipStreamSynchronize(c.s));
+  });
+}
+
+int ki_gmres_resident(ki_ctx* ctx, double tol, int maxit, int restart,
+                      int* iters, double* relres) {
+  return guard([&] {
+    Ctx& c = ctx->c;
+    KI_CHECK(c.factored, KI_E_STATE, "factor first");
+    KI_HIP(hipSetDevice(c.device));
+    c.d_x.ensure(sizeof(double) * c.ndof);
+    double* x = c.d_x.as<double>();
+    ki::solve_dev(c, c.d_b.as<double>(), x);  // x0 = M^{-1} b
+    ki::gmres_dev(c, c.d_b.as<double>(), x, tol, maxit, restart, iters,
+                  relres);
+  });
+}
+
+int ki_get_solution(ki_ctx* ctx, double* x) {
+  return guard([&] {
+    Ctx& c = ctx->c;
+    KI_CHECK(c.d_x.p != nullptr, KI_E_STATE, "no solution");
+    KI_HIP(hipMemcpy(x, c.d_x.p, sizeof(double) * c.ndof,
+                     hipMemcpyDeviceToHost));
+  });
+}
+
+int ki_gmres(ki_ctx* ctx, const double* b, double* x, double tol, int maxit,
+             int restart, int* iters, double* relres) {
+  int rc = ki_set_rhs(ctx, b);
+  if (rc) return rc;
+  rc = ki_gmres_resident(ctx, tol, maxit, restart, iters, relres);
+  if (rc) return rc;
+  return ki_get_solution(ctx, x);
+}
+
+int ki_kernel_block(ki_ctx* ctx, const int* tgt, int nt, const int* src,
+                    int ns, double* out) {
+  return guard([&] {
+    Ctx& c = ctx->c;
+    KI_CHECK(c.npts > 0, KI_E_STATE, "no geometry");
+    KI_CHECK(c.dom == 2 || c.diag_ready, KI_E_STATE,
+             "3D kernels need ki_compute_diag3d first");
+    KI_CHECK(nt >= 0 && ns >= 0, KI_E_ARG, "negative size");
+    for (int i = 0; i < nt; i++)
+      KI_CHECK(tgt[i] >= 0 && tgt[i] < c.ndof, KI_E_ARG, "target out of range");
+    for (int j = 0; j < ns; j++)
+      KI_CHECK(src[j] >= 0 && src[j] < c.ndof, KI_E_ARG, "source out of range");
+    if (nt == 0 || ns == 0) return;
+    KI_HIP(hipSetDevice(c.device));
+    std::vector<int> idx(src, src + ns);
+    idx.insert(idx.end(), tgt, tgt + nt);
+    ki::upload(c.idxb, idx, c.s);
+    ki::FillDesc f{};
+    f.ncols = ns;
+    f.nseg = 1;
+    f.seg_kind[0] = ki::SEG_K;
+    f.seg_rows[0] = nt;
+    f.seg_idx[0] = ns;
+    f.col_idx = 0;
+    f.out_off = 0;
+    f.ld = nt;
+    std::vector<ki::FillDesc> fv{f};
+    ki::upload(c.fdesc, fv, c.s);
+    c.tmp.ensure(sizeof(double) * (size_t)nt * ns);
+    ki::ProxySpec px{};
+    ki::launch_fill<double>(c.g, px, c.fdesc.as<ki::FillDesc>(), 1, ns,
+                            c.idxb.as<int>(), c.tmp.as<double>(), c.s);
+    KI_HIP(hipMemcpyAsync(out, c.tmp.p, sizeof(double) * (size_t)nt * ns,
+                          hipMemcpyDeviceToHost, c.s));
+    KI_HIP(hipStreamSynchronize(c.s));
+  });
+}
+
+int ki_proxy_block(ki_ctx* ctx, const ki_params* params, const double* center,
+                   double radius, const int* box, int n, double* out,
+                   int* rows) {
+  return guard([&] {
+    Ctx& c = ctx->c;
+    KI_CHECK(c.npts > 0, KI_E_STATE, "no geometry");
+    KI_CHECK(params && center && rows && n >= 0, KI_E_ARG, "bad argument");
+    const ki::ProxySpec px = ki::make_proxy(*params, c.dom);
+    const int P = 2 * c.sol * px.npts;
+    *rows = P;
+    if (!out || n == 0) return;
+    for (int j = 0; j < n; j++)
+      KI_CHECK(box[j] >= 0 && box[j] < c.ndof, KI_E_ARG, "DOF out of range");
+    KI_HIP(hipSetDevice(c.device));
+    std::vector<int> idx(box, box + n);
+    ki::upload(c.idxb, idx, c.s);
+    ki::FillDesc f{};
+    f.ncols = n;
+    f.nseg = 1;
+    f.seg_kind[0] = ki::SEG_PROXY;
+    f.seg_rows[0] = P;
+    f.seg_idx[0] = 0;
+    f.col_idx = 0;
+    f.out_off = 0;
+    f.ld = P;
+    for (int q = 0; q < 3; q++) f.center[q] = center[q];
+    f.radius = radius;
+    std::vector<ki::FillDesc> fv{f};
+    ki::upload(c.fdesc, fv, c.s);
+    c.tmp.ensure(sizeof(double) * (size_t)P * n);
+    ki::launch_fill<double>(c.g, px, c.fdesc.as<ki::FillDesc>(), 1, n,
+                            c.idxb.as<int>(), c.tmp.as<double>(), c.s);
+    KI_HIP(hipMemcpyAsync(out, c.tmp.p, sizeof(double) * (size_t)P * n,
+                          hipMemcpyDeviceToHost, c.s));
+    KI_HIP(hipStreamSynchronize(c.s));
+  });
+}
+
+int ki_apply_forward(ki_ctx* ctx, const double* dpts, int ndp,
+                     const double* mu, double* out) {
+  return guard([&] {
+    Ctx& c = ctx->c;
+    KI_CHECK(c.npts > 0, KI_E_STATE, "no geometry");
+    KI_HIP(hipSetDevice(c.device));
+    c.d_dp.ensure(sizeof(double) * (size_t)std::max(ndp, 1) * c.dom);
+    c.d_dout.ensure(sizeof(double) * (size_t)std::max(ndp, 1) * c.sol);
+    c.d_tmpv.ensure(sizeof(double) * c.ndof);
+    if (ndp <= 0) return;
+    KI_HIP(hipMemcpyAsync(c.d_dp.p, dpts, sizeof(double) * ndp * c.dom,
+                          hipMemcpyHostToDevice, c.s));
+    KI_HIP(hipMemcpyAsync(c.d_tmpv.p, mu, sizeof(double) * c.ndof,
+                          hipMemcpyHostToDevice, c.s));
+    ki::launch_forward(c.g, c.d_dp.as<double>(), ndp, c.d_tmpv.as<double>(),
+                       c.d_dout.as<double>(), c.s);
+    KI_HIP(hipMemcpyAsync(out, c.d_dout.p, sizeof(double) * ndp * c.sol,
+                          hipMemcpyDeviceToHost, c.s));
+    KI_HIP(hipStreamSynchronize(c.s));
+  });
+}
+
+int ki_get_stats(ki_ctx* ctx, ki_stats* out) {
+  return guard([&] {
+    KI_CHECK(out != nullptr, KI_E_ARG, "null output");
+    *out = ctx->c.stats;
+  });
+}
+
+int ki_box_log(ki_ctx* ctx, int* out, int cap, int* count) {
+  return guard([&] {
+    const auto& v = ctx->c.boxlog;
+    const int n = v.size() / 4;
+    if (count) *count = n;
+    const int nc = std::min(cap, n);
+    if (out && nc > 0) std::memcpy(out, v.data(), sizeof(int) * 4 * nc);
+  });
+}
+
+}  // extern "C"

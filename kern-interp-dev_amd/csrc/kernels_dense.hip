@@ -1,0 +1,578 @@
+// Batched dense kernels on variable-size matrices: GEMM, partial-pivoted LU,
+// LU solves (LAPACK getrf/getrs semantics), copies and the Schur-update
+// scatter.  Every launch covers all boxes of a tree level; a box's matrices
+// are addressed by element offsets into one buffer per level, so the same
+// kernels serve one 64x64 leaf and one 10,000-row top block.
+//
+// These replace the reference's per-box cblas_dgemm / dgetrf / dgetrs calls
+// (ki_mat.cpp:328-335, 445-513) inside decouple (skel_factorization.cpp:
+// 91-127), the top-level LU (:285-310) and the block-diagonal solves of the
+// sweep (:585-592, 676-722).
+#include <algorithm>
+#include <cmath>
+
+#include "ki_internal.h"
+
+namespace ki {
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// GEMM: C = alpha op(A) op(B) + beta C, 64x64 output tile per workgroup,
+// 256 threads with a 4x4 register micro-tile, K staged through LDS.
+// ---------------------------------------------------------------------------
+constexpr int GT = 64, GK = 16;
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+gemm_kernel(const GemmDesc* __restrict__ desc, const T* __restrict__ a_all,
+            const T* __restrict__ b_all, T* __restrict__ c_all) {
+  const GemmDesc d = desc[blockIdx.y];
+  const int tiles_m = (d.m + GT - 1) / GT;
+  const int tiles_n = (d.n + GT - 1) / GT;
+  if ((int)blockIdx.x >= tiles_m * tiles_n) return;
+  const int tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
+  const int i0 = tm * GT, j0 = tn * GT;
+  const T* A = a_all + d.a_off;
+  const T* B = b_all + d.b_off;
+  T* C = c_all + d.c_off;
+  __shared__ T As[GK][GT + 1];
+  __shared__ T Bs[GK][GT + 1];
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  T acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[a][b] = 0;
+  for (int k0 = 0; k0 < d.k; k0 += GK) {
+    // A tile: rows i0..i0+63 of op(A), cols k0..k0+15
+    for (int e = threadIdx.x; e < GT * GK; e += 256) {
+      int ii, kk;
+      if (!d.ta) {
+        ii = e % GT;
+        kk = e / GT;
+      } else {
+        kk = e % GK;
+        ii = e / GK;
+      }
+      const int gi = i0 + ii, gk = k0 + kk;
+      T v = 0;
+      if (gi < d.m && gk < d.k)
+        v = d.ta ? A[gk + (int64_t)d.lda * gi] : A[gi + (int64_t)d.lda * gk];
+      As[kk][ii] = v;
+    }
+    // B tile: rows k0..k0+15 of op(B), cols j0..j0+63
+    for (int e = threadIdx.x; e < GT * GK; e += 256) {
+      int kk, jj;
+      if (!d.tb) {
+        kk = e % GK;
+        jj = e / GK;
+      } else {
+        jj = e % GT;
+        kk = e / GT;
+      }
+      const int gk = k0 + kk, gj = j0 + jj;
+      T v = 0;
+      if (gk < d.k && gj < d.n)
+        v = d.tb ? B[gj + (int64_t)d.ldb * gk] : B[gk + (int64_t)d.ldb * gj];
+      Bs[kk][jj] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GK; kk++) {
+      T av[4], bv[4];
+#pragma unroll
+      for (int a = 0; a < 4; a++) av[a] = As[kk][tx + 16 * a];
+#pragma unroll
+      for (int b = 0; b < 4; b++) bv[b] = Bs[kk][ty + 16 * b];
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] += av[a] * bv[b];
+    }
+    __syncthreads();
+  }
+  const T alpha = (T)d.alpha, beta = (T)d.beta;
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    const int gj = j0 + ty + 16 * b;
+    if (gj >= d.n) continue;
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      const int gi = i0 + tx + 16 * a;
+      if (gi >= d.m) continue;
+      T* cp = C + gi + (int64_t)d.ldc * gj;
+      *cp = beta == (T)0 ? alpha * acc[a][b] : alpha * acc[a][b] + beta * *cp;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LU with partial pivoting (getrf), right-looking, panel width LB.
+// ---------------------------------------------------------------------------
+constexpr int LB = 32;
+
+template <typename T>
+__device__ __forceinline__ T wsum(T v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Factor the panel A[j0:m, j0:j0+jb] of each box (one workgroup per box).
+template <typename T>
+__global__ void __launch_bounds__(256)
+lu_panel(const MatDesc* __restrict__ desc, T* __restrict__ a_all,
+         int* __restrict__ piv_all, int j0) {
+  const MatDesc d = desc[blockIdx.x];
+  const int m = d.m, n = d.n;
+  if (j0 >= min(m, n)) return;
+  const int jb = min(LB, min(m, n) - j0);
+  T* A = a_all + d.off;
+  int* piv = piv_all + d.piv_off;
+  const int ld = d.ld;
+  __shared__ T rv[4];
+  __shared__ int ri[4];
+  __shared__ int s_p;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int jj = 0; jj < jb; jj++) {
+    const int c = j0 + jj;
+    T* col = A + (int64_t)ld * c;
+    // pivot: first row of maximal |a| (idamax)
+    T best = -1;
+    int bi = m;
+    for (int r = c + threadIdx.x; r < m; r += 256) {
+      const T v = fabs(col[r]);
+      if (v > best) {
+        best = v;
+        bi = r;
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      const T ov = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      if (ov > best || (ov == best && oi < bi)) {
+        best = ov;
+        bi = oi;
+      }
+    }
+    if (lane == 0) {
+      rv[wid] = best;
+      ri[wid] = bi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      T b = rv[0];
+      int i = ri[0];
+      for (int w = 1; w < 4; w++)
+        if (rv[w] > b || (rv[w] == b && ri[w] < i)) {
+          b = rv[w];
+          i = ri[w];
+        }
+      if (i < c || i >= m) i = c;  // non-finite column: no interchange
+      s_p = i;
+      piv[c] = i;
+    }
+    __syncthreads();
+    const int p = s_p;
+    if (p != c) {  // swap rows c and p inside the panel
+      for (int q = threadIdx.x; q < jb; q += 256) {
+        T* cc = A + (int64_t)ld * (j0 + q);
+        const T t = cc[c];
+        cc[c] = cc[p];
+        cc[p] = t;
+      }
+    }
+    __syncthreads();
+    const T pv = col[c];
+    if (pv != (T)0) {
+      const T inv = (T)1 / pv;
+      for (int r = c + 1 + threadIdx.x; r < m; r += 256) col[r] *= inv;
+    }
+    __syncthreads();
+    // rank-1 update of the rest of the panel
+    for (int q = jj + 1 + wid; q < jb; q += 4) {
+      T* cq = A + (int64_t)ld * (j0 + q);
+      const T u = cq[c];
+      for (int r = c + 1 + lane; r < m; r += 64) cq[r] -= col[r] * u;
+    }
+    __syncthreads();
+  }
+}
+
+// Apply the panel's row interchanges to the columns outside the panel, then
+// A12 <- L11^{-1} A12 for the columns to the right (one thread per column).
+template <typename T>
+__global__ void __launch_bounds__(256)
+lu_swap_trsm(const MatDesc* __restrict__ desc, T* __restrict__ a_all,
+             const int* __restrict__ piv_all, int j0) {
+  const MatDesc d = desc[blockIdx.y];
+  const int m = d.m, n = d.n;
+  if (j0 >= min(m, n)) return;
+  const int jb = min(LB, min(m, n) - j0);
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ncols_out = n - jb;
+  if (col >= ncols_out) return;
+  const int c = col < j0 ? col : col + jb;
+  T* A = a_all + d.off;
+  const int* piv = piv_all + d.piv_off;
+  T* cc = A + (int64_t)d.ld * c;
+  for (int q = 0; q < jb; q++) {
+    const int r = j0 + q, p = piv[r];
+    if (p != r) {
+      const T t = cc[r];
+      cc[r] = cc[p];
+      cc[p] = t;
+    }
+  }
+  if (c >= j0 + jb) {
+    for (int q = 0; q < jb; q++) {
+      const T x = cc[j0 + q];
+      for (int t = q + 1; t < jb; t++)
+        cc[j0 + t] -= A[(j0 + t) + (int64_t)d.ld * (j0 + q)] * x;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Triangular solves with the LU factors, blocked by LB rows:
+//   mode 0: L (unit lower) forward      mode 1: U (upper) backward
+//   mode 2: U^T (lower) forward         mode 3: L^T (unit upper) backward
+// The diagonal-block kernel solves one LB block for every right-hand side
+// column (one thread per column); GEMM updates the remaining rows.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256)
+trsm_diag(const MatDesc* __restrict__ desc, const T* __restrict__ a_all,
+          T* __restrict__ b_all, int blk, int mode) {
+  const MatDesc d = desc[blockIdx.y];
+  const int n = d.n;
+  const int nblk = (n + LB - 1) / LB;
+  if (blk >= nblk) return;
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= d.nrhs) return;
+  // forward modes walk blocks top-down, backward modes bottom-up
+  const int b = (mode == 0 || mode == 2) ? blk : nblk - 1 - blk;
+  const int r0 = b * LB, rb = min(LB, n - r0);
+  const T* A = a_all + d.off;
+  const int ld = d.ld;
+  T* x = b_all + d.b_off + (int64_t)d.ldb * col;
+  if (mode == 0) {
+    for (int q = 0; q < rb; q++) {
+      const T v = x[r0 + q];
+      for (int t = q + 1; t < rb; t++)
+        x[r0 + t] -= A[(r0 + t) + (int64_t)ld * (r0 + q)] * v;
+    }
+  } else if (mode == 1) {
+    for (int q = rb - 1; q >= 0; q--) {
+      const T v = x[r0 + q] / A[(r0 + q) + (int64_t)ld * (r0 + q)];
+      x[r0 + q] = v;
+      for (int t = 0; t < q; t++)
+        x[r0 + t] -= A[(r0 + t) + (int64_t)ld * (r0 + q)] * v;
+    }
+  } else if (mode == 2) {  // U^T lower: (U^T)(t,q) = U(q,t)
+    for (int q = 0; q < rb; q++) {
+      const T v = x[r0 + q] / A[(r0 + q) + (int64_t)ld * (r0 + q)];
+      x[r0 + q] = v;
+      for (int t = q + 1; t < rb; t++)
+        x[r0 + t] -= A[(r0 + q) + (int64_t)ld * (r0 + t)] * v;
+    }
+  } else {  // L^T unit upper: (L^T)(t,q) = L(q,t)
+    for (int q = rb - 1; q >= 0; q--) {
+      const T v = x[r0 + q];
+      for (int t = 0; t < q; t++)
+        x[r0 + t] -= A[(r0 + q) + (int64_t)ld * (r0 + t)] * v;
+    }
+  }
+}
+
+// Row interchanges on a right-hand-side block (laswp), forward or reverse.
+template <typename T>
+__global__ void __launch_bounds__(256)
+rhs_swap(const MatDesc* __restrict__ desc, T* __restrict__ b_all,
+         const int* __restrict__ piv_all, int reverse) {
+  const MatDesc d = desc[blockIdx.y];
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= d.nrhs) return;
+  const int* piv = piv_all + d.piv_off;
+  T* x = b_all + d.b_off + (int64_t)d.ldb * col;
+  const int n = min(d.m, d.n);
+  if (!reverse) {
+    for (int r = 0; r < n; r++) {
+      const int p = piv[r];
+      if (p != r) {
+        const T t = x[r];
+        x[r] = x[p];
+        x[p] = t;
+      }
+    }
+  } else {
+    for (int r = n - 1; r >= 0; r--) {
+      const int p = piv[r];
+      if (p != r) {
+        const T t = x[r];
+        x[r] = x[p];
+        x[p] = t;
+      }
+    }
+  }
+}
+
+// Copy (optionally transposed) m x n blocks: dst = op(src).
+template <typename T>
+__global__ void __launch_bounds__(256)
+copy_kernel(const MatDesc* __restrict__ desc, const T* __restrict__ src,
+            T* __restrict__ dst, int transpose) {
+  const MatDesc d = desc[blockIdx.y];
+  const int64_t total = (int64_t)d.m * d.n;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = e % d.m, j = e / d.m;
+    const T v = src[d.off + i + (int64_t)d.ld * j];
+    if (!transpose)
+      dst[d.dst_off + i + (int64_t)d.ldd * j] = v;
+    else
+      dst[d.dst_off + j + (int64_t)d.ldd * i] = v;
+  }
+}
+
+template <typename T>
+__global__ void zero_kernel(T* p, int64_t n) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x)
+    p[e] = 0;
+}
+
+template <typename TS, typename TD>
+__global__ void __launch_bounds__(256)
+scatter_kernel(const ScatterDesc* __restrict__ desc, const TS* __restrict__ src_all,
+               const int* __restrict__ map, TD* __restrict__ dst, TD sign,
+               TD* __restrict__ acc) {
+  const ScatterDesc d = desc[blockIdx.y];
+  const int j = blockIdx.x;
+  if (j >= d.n) return;
+  const TS* src = src_all + d.src_off + (int64_t)d.n * j;
+  const int* mp = map + d.map_off;
+  const int pj = mp[j];
+  for (int i = threadIdx.x; i < d.n; i += blockDim.x) {
+    const int pi = mp[i];
+    const TD v = (TD)src[i];
+    if (d.dst_off >= 0) dst[d.dst_off + pi + (int64_t)d.ld * pj] += sign * v;
+    if (d.acc_off >= 0 && pi < d.acc_k && pj < d.acc_k)
+      acc[d.acc_off + pi + (int64_t)d.acc_ld * pj] += v;
+  }
+}
+
+}  // namespace
+
+template <typename T>
+void launch_gemm(const GemmDesc* d_desc, int ndesc, int max_m, int max_n,
+                 const T* a, const T* b, T* c, hipStream_t s) {
+  if (ndesc == 0 || max_m == 0 || max_n == 0) return;
+  const int tiles = ((max_m + GT - 1) / GT) * ((max_n + GT - 1) / GT);
+  for (int base = 0; base < ndesc; base += 65535) {
+    const int cnt = std::min(65535, ndesc - base);
+    hipLaunchKernelGGL(gemm_kernel<T>, dim3(tiles, cnt), dim3(256), 0, s,
+                       d_desc + base, a, b, c);
+  }
+  KI_HIP(hipGetLastError());
+}
+
+template <typename T>
+void batched_lu(const std::vector<MatDesc>& h, MatDesc* d_desc, T* a, int* piv,
+                hipStream_t s) {
+  if (h.empty()) return;
+  int maxmn = 0, maxn = 0;
+  for (auto& x : h) {
+    maxmn = std::max(maxmn, std::min(x.m, x.n));
+    maxn = std::max(maxn, x.n);
+  }
+  const int nb = h.size();
+  const int npan = (maxmn + LB - 1) / LB;
+  // Trailing-update GEMM descriptors for every panel, uploaded once.
+  std::vector<GemmDesc> gd((size_t)nb * npan);
+  std::vector<int> mm(npan, 0), nn(npan, 0);
+  for (int p = 0; p < npan; p++) {
+    const int j0 = p * LB;
+    for (int b = 0; b < nb; b++) {
+      const MatDesc& x = h[b];
+      const int lim = std::min(x.m, x.n);
+      GemmDesc& g = gd[(size_t)p * nb + b];
+      g = GemmDesc{0, 0, 0, 0, 0, 0, 1, 1, 1, 0, 0, -1.0, 1.0};
+      if (j0 >= lim) continue;
+      const int jb = std::min(LB, lim - j0);
+      g.m = x.m - j0 - jb;
+      g.n = x.n - j0 - jb;
+      if (g.m <= 0 || g.n <= 0) {
+        g.m = g.n = 0;
+        continue;
+      }
+      g.k = jb;
+      g.a_off = x.off + (j0 + jb) + (int64_t)x.ld * j0;
+      g.b_off = x.off + j0 + (int64_t)x.ld * (j0 + jb);
+      g.c_off = x.off + (j0 + jb) + (int64_t)x.ld * (j0 + jb);
+      g.lda = g.ldb = g.ldc = x.ld;
+      mm[p] = std::max(mm[p], g.m);
+      nn[p] = std::max(nn[p], g.n);
+    }
+  }
+  GemmDesc* d_g = nullptr;
+  KI_HIP(hipMallocAsync((void**)&d_g, sizeof(GemmDesc) * gd.size(), s));
+  KI_HIP(hipMemcpyAsync(d_g, gd.data(), sizeof(GemmDesc) * gd.size(),
+                        hipMemcpyHostToDevice, s));
+  for (int p = 0; p < npan; p++) {
+    const int j0 = p * LB;
+    hipLaunchKernelGGL(lu_panel<T>, dim3(nb), dim3(256), 0, s, d_desc, a, piv,
+                       j0);
+    hipLaunchKernelGGL(lu_swap_trsm<T>, dim3((maxn + 255) / 256, nb),
+                       dim3(256), 0, s, d_desc, a, piv, j0);
+    if (mm[p] > 0 && nn[p] > 0)
+      launch_gemm<T>(d_g + (size_t)p * nb, nb, mm[p], nn[p], a, a, a, s);
+  }
+  KI_HIP(hipGetLastError());
+  // the host descriptor vector dies here: make sure the upload has landed
+  KI_HIP(hipStreamSynchronize(s));
+  KI_HIP(hipFreeAsync(d_g, s));
+}
+
+template <typename T>
+void batched_getrs(const std::vector<MatDesc>& h, MatDesc* d_desc, const T* a,
+                   const int* piv, T* b, bool trans, hipStream_t s) {
+  if (h.empty()) return;
+  const int nb = h.size();
+  int maxn = 0, maxr = 0;
+  for (auto& x : h) {
+    maxn = std::max(maxn, x.n);
+    maxr = std::max(maxr, x.nrhs);
+  }
+  if (maxr == 0) return;
+  const int nblk = (maxn + LB - 1) / LB;
+  const int modes[2] = {trans ? 2 : 0, trans ? 3 : 1};
+  // GEMM descriptors for both triangular sweeps, every block, uploaded once.
+  std::vector<GemmDesc> gd((size_t)2 * nblk * nb);
+  std::vector<int> mm(2 * nblk, 0), nn(2 * nblk, 0);
+  for (int mi = 0; mi < 2; mi++) {
+    const int mode = modes[mi];
+    const bool fwd = (mode == 0 || mode == 2);
+    for (int blk = 0; blk < nblk; blk++) {
+      const int slot = mi * nblk + blk;
+      for (int q = 0; q < nb; q++) {
+        const MatDesc& x = h[q];
+        const int xb = (x.n + LB - 1) / LB;
+        GemmDesc& g = gd[(size_t)slot * nb + q];
+        g = GemmDesc{0, 0, 0, 0, 0, 0, 1, 1, 1, 0, 0, -1.0, 1.0};
+        if (blk >= xb || x.nrhs == 0) continue;
+        const int bb = fwd ? blk : xb - 1 - blk;
+        const int r0 = bb * LB, rb = std::min(LB, x.n - r0);
+        g.n = x.nrhs;
+        g.k = rb;
+        g.b_off = x.b_off + r0;
+        g.ldb = x.ldb;
+        g.lda = x.ld;
+        g.ldc = x.ldb;
+        if (fwd) {  // rows below the block
+          g.m = x.n - r0 - rb;
+          g.c_off = x.b_off + r0 + rb;
+          if (mode == 0) {
+            g.a_off = x.off + (r0 + rb) + (int64_t)x.ld * r0;
+            g.ta = 0;
+          } else {  // U^T rows r0+rb.., cols r0.. = U(r0.., r0+rb..)^T
+            g.a_off = x.off + r0 + (int64_t)x.ld * (r0 + rb);
+            g.ta = 1;
+          }
+        } else {  // rows above the block
+          g.m = r0;
+          g.c_off = x.b_off;
+          if (mode == 1) {
+            g.a_off = x.off + (int64_t)x.ld * r0;
+            g.ta = 0;
+          } else {  // L^T rows 0..r0, cols r0.. = L(r0.., 0..r0)^T
+            g.a_off = x.off + r0;
+            g.ta = 1;
+          }
+        }
+        if (g.m <= 0) {
+          g.m = g.n = 0;
+          continue;
+        }
+        mm[slot] = std::max(mm[slot], g.m);
+        nn[slot] = std::max(nn[slot], g.n);
+      }
+    }
+  }
+  GemmDesc* d_g = nullptr;
+  KI_HIP(hipMallocAsync((void**)&d_g, sizeof(GemmDesc) * gd.size(), s));
+  KI_HIP(hipMemcpyAsync(d_g, gd.data(), sizeof(GemmDesc) * gd.size(),
+                        hipMemcpyHostToDevice, s));
+  const dim3 cgrid((maxr + 255) / 256, nb);
+  if (!trans)
+    hipLaunchKernelGGL(rhs_swap<T>, cgrid, dim3(256), 0, s, d_desc, b, piv, 0);
+  for (int mi = 0; mi < 2; mi++) {
+    for (int blk = 0; blk < nblk; blk++) {
+      const int slot = mi * nblk + blk;
+      hipLaunchKernelGGL(trsm_diag<T>, cgrid, dim3(256), 0, s, d_desc, a, b,
+                         blk, modes[mi]);
+      if (mm[slot] > 0 && nn[slot] > 0)
+        launch_gemm<T>(d_g + (size_t)slot * nb, nb, mm[slot], nn[slot], a, b,
+                       b, s);
+    }
+  }
+  if (trans)
+    hipLaunchKernelGGL(rhs_swap<T>, cgrid, dim3(256), 0, s, d_desc, b, piv, 1);
+  KI_HIP(hipGetLastError());
+  KI_HIP(hipStreamSynchronize(s));
+  KI_HIP(hipFreeAsync(d_g, s));
+}
+
+template <typename T>
+void launch_copy(const MatDesc* d_desc, int ndesc, int max_m, int max_n,
+                 const T* src, T* dst, int transpose, hipStream_t s) {
+  if (ndesc == 0) return;
+  const int64_t tot = (int64_t)max_m * max_n;
+  const int gx = (int)std::min<int64_t>((tot + 255) / 256, 4096);
+  if (gx == 0) return;
+  hipLaunchKernelGGL(copy_kernel<T>, dim3(gx, ndesc), dim3(256), 0, s, d_desc,
+                     src, dst, transpose);
+  KI_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_zero(T* p, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  const int gx = (int)std::min<int64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(zero_kernel<T>, dim3(gx), dim3(256), 0, s, p, n);
+  KI_HIP(hipGetLastError());
+}
+
+template <typename TS, typename TD>
+void launch_scatter(const ScatterDesc* d_desc, int ndesc, int max_n,
+                    const TS* src, const int* d_map, TD* dst, TD sign,
+                    TD* acc, hipStream_t s) {
+  if (ndesc == 0 || max_n == 0) return;
+  for (int base = 0; base < ndesc; base += 65535) {
+    const int cnt = std::min(65535, ndesc - base);
+    hipLaunchKernelGGL((scatter_kernel<TS, TD>), dim3(max_n, cnt), dim3(256),
+                       0, s, d_desc + base, src, d_map, dst, sign, acc);
+  }
+  KI_HIP(hipGetLastError());
+}
+
+#define KI_INST(T)                                                            \
+  template void launch_gemm<T>(const GemmDesc*, int, int, int, const T*,      \
+                               const T*, T*, hipStream_t);                    \
+  template void batched_lu<T>(const std::vector<MatDesc>&, MatDesc*, T*,      \
+                              int*, hipStream_t);                             \
+  template void batched_getrs<T>(const std::vector<MatDesc>&, MatDesc*,       \
+                                 const T*, const int*, T*, bool, hipStream_t); \
+  template void launch_copy<T>(const MatDesc*, int, int, int, const T*, T*,   \
+                               int, hipStream_t);                             \
+  template void launch_zero<T>(T*, int64_t, hipStream_t);                     \
+  template void launch_scatter<T, T>(const ScatterDesc*, int, int, const T*, \
+                                     const int*, T*, T, T*, hipStream_t);
+KI_INST(float)
+KI_INST(double)
+#undef KI_INST
+template void launch_scatter<float, double>(const ScatterDesc*, int, int,
+                                            const float*, const int*, double*,
+                                            double, double*, hipStream_t);
+
+}  // namespace ki

@@ -1,0 +1,256 @@
+// Internal declarations shared by the engine's host code and HIP kernels.
+// Nothing here is part of the public C-ABI (include/ki_abi.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+namespace ki {
+
+// ---------------------------------------------------------------------------
+// Error handling: every HIP call is checked; failures throw ki::Error, which
+// the C-ABI layer turns into a status code plus ki_last_error() text.
+// ---------------------------------------------------------------------------
+struct Error {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string& msg);
+
+#define KI_HIP(expr)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (expr);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      ::ki::fail(3, std::string(#expr) + ": " + hipGetErrorString(e_) +     \
+                        " at " + __FILE__ + ":" + std::to_string(__LINE__)); \
+  } while (0)
+
+#define KI_CHECK(cond, code, msg)                 \
+  do {                                            \
+    if (!(cond)) ::ki::fail((code), (msg));       \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Tree (host side, bit-exact restatement of the reference's QuadTree build).
+// ---------------------------------------------------------------------------
+struct Box {
+  int level = 0;
+  int parent = -1;
+  int index_in_level = 0;
+  bool leaf = true;
+  double center[3] = {0, 0, 0};
+  double side = 0;
+  std::vector<int> children;   // box ids, 2^d of them when not a leaf
+  std::vector<int> neighbors;  // box ids, in the reference's list order
+  std::vector<int> original;   // DOFs below this box, ascending
+  std::vector<int> active;     // DOFs still active when the level runs
+  std::vector<int> near;       // near-field DOFs (see remove_inactive)
+  std::vector<int> skel, redundant;
+  std::vector<int> perm;       // local ID permutation (skel first)
+  bool compressed = false;
+  int k = 0, r = 0;            // skeleton rank, redundant count
+  // Device factor data (element offsets into the level's buffers).
+  int fac_level = -1;          // which LevelFactor holds this box's data
+  int64_t offT = -1, offL = -1, offU = -1, offX = -1, offPiv = -1;
+  int64_t offAcc = -1;         // accumulated Schur update on output DOFs
+  int acc_n = 0;               // size of the output DOF set holding Acc
+  // Features for the rank predictor (filled when the box is ID'ed).
+  int id_rows = 0;
+};
+
+struct Tree {
+  int dom = 2, sol = 1, cap = 64;
+  std::vector<Box> boxes;
+  std::vector<std::vector<int>> levels;  // box ids per level, reference order
+  int root = 0;
+  void build(const double* pts, int npts, int dom, int sol, int cap);
+  // quadtree.cpp:1142-1204 / 1236-1261
+  void remove_inactive_dofs_at_level(int level);
+  void remove_inactive_dofs_at_all_boxes();
+
+ private:
+  const double* pts_ = nullptr;
+  int child_idx(const Box& b, const double* p) const;
+  void recursive_add(int box, const double* p, int point_ind);
+  void subdivide(int box);
+  void compute_neighbor_lists();
+  void descendent_neighbors(int big, int small);
+  void remove_inactive_dofs_at_box(int box);
+};
+
+// ---------------------------------------------------------------------------
+// Geometry on the device (FP64 master copy plus FP32 copies of the
+// quantities that the FP32 generator reads).
+// ---------------------------------------------------------------------------
+struct DevGeom {
+  int npts = 0, dom = 2, sol = 1, pde = 0;
+  double* pts = nullptr;    // npts*dom
+  double* nrm = nullptr;    // npts*dom
+  double* w = nullptr;      // npts
+  double* curv = nullptr;   // npts
+  double* diag = nullptr;   // 3D: npts (Laplace) or npts*9 (Stokes)
+};
+
+// Kernel-entry evaluation mode for one row segment of a generated block.
+enum SegKind : int {
+  SEG_K = 0,       // entry (i,j) = K(rows[i], cols[j])
+  SEG_KT = 1,      // entry (i,j) = K(cols[j], rows[i])   (transposed block)
+  SEG_PROXY = 2,   // proxy rows: [box->proxy ; proxy->box]
+};
+
+// Proxy surface description.
+struct ProxySpec {
+  int npts;        // number of proxy points
+  int kind;        // 0 = 2D equispaced circle / 3D GL tensor, 1 = Fibonacci
+  int ntheta, nphi;
+  double gl_phi[64], gl_w[64];  // GL nodes/weights on [0, pi] (3D tensor)
+};
+
+// One generated block: up to three row segments stacked vertically, all with
+// the same column DOF list.  Column-major output with leading dimension ld.
+struct FillDesc {
+  int ncols;
+  int nseg;
+  int seg_kind[3];
+  int seg_rows[3];         // row count of each segment
+  int64_t seg_idx[3];      // offset of the segment's DOF list in idx buffer
+  int64_t col_idx;         // offset of the column DOF list in idx buffer
+  int64_t out_off;         // element offset of the block in the output
+  int ld;
+  double center[3];        // proxy centre (SEG_PROXY)
+  double radius;           // proxy radius (SEG_PROXY)
+};
+
+template <typename T>
+void launch_fill(const DevGeom& g, const ProxySpec& px, const FillDesc* d_desc,
+                 int ndesc, int max_cols, const int* d_idx, T* out,
+                 hipStream_t s);
+
+// Scatter-add jobs: dst[map[i] + ld*map[j]] += sign * src[i + n*j] for an
+// n x n source; with acc, entries whose mapped row and column are both
+// below acc_k are also added (with +1) to acc (leading dim acc_ld).
+struct ScatterDesc {
+  int n;
+  int64_t src_off;   // offset of the n x n source block
+  int64_t map_off;   // offset into the map buffer
+  int64_t dst_off;   // -1 when unused
+  int ld;
+  int64_t acc_off;   // -1 when unused
+  int acc_ld;
+  int acc_k;
+};
+
+template <typename TS, typename TD>
+void launch_scatter(const ScatterDesc* d_desc, int ndesc, int max_n,
+                    const TS* src, const int* d_map, TD* dst, TD sign,
+                    TD* acc, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Batched CPQR (interpolative decomposition).
+// ---------------------------------------------------------------------------
+struct QrDesc {
+  int m, n;
+  int64_t a_off;     // matrix (ld = m)
+  int64_t piv_off;   // n ints: local column permutation (output)
+  int64_t ws_off;    // workspace: 4*n elements of T (norms, dots)
+  int max_rank;      // stop after this many steps (n normally)
+};
+
+template <typename T>
+void launch_cpqr(const QrDesc* d_desc, int ndesc, int max_n, T* a, int* piv,
+                 T* ws, int* rank, double tol, hipStream_t s);
+
+// Multi-workgroup CPQR for a few large boxes (per-step launches).
+template <typename T>
+void cpqr_large(const std::vector<QrDesc>& desc, T* a, int* piv, T* ws,
+                int* d_rank, int* h_rank, double tol, hipStream_t s);
+
+// T = R11^{-1} R12 in place (rows 0..k-1, cols k..n-1 of the QR output).
+template <typename T>
+void launch_idsolve(const QrDesc* d_desc, int ndesc, int max_n, T* a,
+                    const int* rank, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Batched dense kernels: GEMM, LU, triangular solves, copies.
+// ---------------------------------------------------------------------------
+struct GemmDesc {
+  int m, n, k;
+  int64_t a_off, b_off, c_off;
+  int lda, ldb, ldc;
+  int ta, tb;        // 1 = transposed operand
+  double alpha, beta;
+};
+
+template <typename T>
+void launch_gemm(const GemmDesc* d_desc, int ndesc, int max_m, int max_n,
+                 const T* a, const T* b, T* c, hipStream_t s);
+
+struct MatDesc {       // one square (or m x n) matrix for LU / TRSM / copy
+  int m, n;
+  int64_t off;
+  int ld;
+  int64_t piv_off;
+  int64_t b_off;       // right-hand side block (TRSM)
+  int ldb, nrhs;
+  int64_t dst_off;     // copy destination
+  int ldd;
+};
+
+template <typename T>
+void batched_lu(const std::vector<MatDesc>& h, MatDesc* d_desc, T* a,
+                int* piv, hipStream_t s);
+
+// Solve with the LU factors: op(A) X = B, A = P L U (LAPACK getrs).
+template <typename T>
+void batched_getrs(const std::vector<MatDesc>& h, MatDesc* d_desc,
+                   const T* a, const int* piv, T* b, bool trans,
+                   hipStream_t s);
+
+template <typename T>
+void launch_copy(const MatDesc* d_desc, int ndesc, int max_m, int max_n,
+                 const T* src, T* dst, int transpose, hipStream_t s);
+
+template <typename T>
+void launch_zero(T* p, int64_t n, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Solve sweeps and matrix-free operators.
+// ---------------------------------------------------------------------------
+struct SweepDesc {
+  int k, r;
+  int64_t s_idx, r_idx;     // DOF lists (skel / redundant) in idx buffer
+  int64_t offT, offL, offU; // factor offsets
+  int buf;                  // factor buffer index (level)
+  int64_t offX, offPiv;
+};
+
+template <typename T>
+void launch_sweep(const SweepDesc* d, int nd, const int* idx,
+                  T* const* facbufs, T* x, int up, hipStream_t s, int max_n);
+
+template <typename T>
+void launch_diag_solve(const SweepDesc* d, int nd, const int* idx,
+                       T* const* facbufs, int* const* pivbufs, T* x,
+                       hipStream_t s, int max_r);
+
+void launch_matvec(const DevGeom& g, const double* mu, double* y,
+                   hipStream_t s);
+void launch_forward(const DevGeom& g, const double* tpts, int ntgt,
+                    const double* mu, double* y, hipStream_t s);
+void launch_diag3d(DevGeom& g, hipStream_t s);
+
+template <typename TO, typename TI>
+void launch_convert(const TI* in, TO* out, int64_t n, hipStream_t s);
+template <typename TI, typename TO>
+void launch_gather(const TI* x, const int* idx, int n, TO* out,
+                   hipStream_t s);
+template <typename TI, typename TO>
+void launch_scatter_vec(const TI* in, const int* idx, int n, TO* x,
+                        hipStream_t s);
+
+}  // namespace ki

@@ -1,0 +1,193 @@
+// Host-side quad/octree: a from-scratch restatement of the reference's
+// QuadTree build and DOF bookkeeping.  This is integer / compare work and must
+// reproduce the reference's partition bit for bit, including the order of
+// boxes within a level and of entries in every neighbor list, because those
+// orders fix the row order of each box's ID matrix.
+//
+//   build / recursive_add / subdivide   quadtree/quadtree.cpp:240-287,317-413
+//   neighbor lists                      quadtree/quadtree.cpp:192-237,292-314
+//   active / near DOF lists             quadtree/quadtree.cpp:1134-1204,
+//                                       1236-1261 (sorted active lists)
+#include <algorithm>
+#include <cmath>
+
+#include "ki_internal.h"
+
+namespace ki {
+
+int Tree::child_idx(const Box& b, const double* p) const {
+  // child index bit 2^(d-1-i) is set when p_i >= centre_i
+  int ci = 0;
+  for (int i = 0; i < dom; i++)
+    if (p[i] >= b.center[i]) ci += 1 << (dom - i - 1);
+  return ci;
+}
+
+void Tree::build(const double* pts, int npts, int dom_, int sol_, int cap_) {
+  KI_CHECK(npts > 0, 2, "tree needs at least one point");
+  KI_CHECK(dom_ == 2 || dom_ == 3, 2, "domain dimension must be 2 or 3");
+  dom = dom_;
+  sol = sol_;
+  cap = cap_;
+  pts_ = pts;
+  boxes.clear();
+  levels.clear();
+  double mn = pts[0], mx = pts[0];
+  for (int64_t i = 0; i < (int64_t)npts * dom; i++) {
+    if (pts[i] < mn) mn = pts[i];
+    if (pts[i] > mx) mx = pts[i];
+  }
+  const double tmin = mn - 0.01, tmax = mx + 0.01;
+  Box root;
+  root.level = 0;
+  for (int i = 0; i < dom; i++) root.center[i] = (tmin + tmax) / 2.0;
+  root.side = tmax - tmin;
+  boxes.push_back(root);
+  levels.push_back({0});
+  this->root = 0;
+  for (int p = 0; p < npts; p++) recursive_add(0, pts + (int64_t)p * dom, p);
+  compute_neighbor_lists();
+  for (auto& lv : levels)
+    for (size_t k = 0; k < lv.size(); k++) boxes[lv[k]].index_in_level = k;
+  pts_ = nullptr;
+}
+
+void Tree::recursive_add(int b, const double* p, int point_ind) {
+  for (;;) {
+    for (int i = 0; i < sol; i++) boxes[b].original.push_back(sol * point_ind + i);
+    if (!boxes[b].leaf) {
+      b = boxes[b].children[child_idx(boxes[b], p)];
+      continue;
+    }
+    if ((int)boxes[b].original.size() > cap) subdivide(b);
+    return;
+  }
+}
+
+void Tree::subdivide(int b) {
+  boxes[b].leaf = false;
+  const int nch = 1 << dom;
+  const int first = boxes.size();
+  for (int c = 0; c < nch; c++) {
+    Box ch;
+    int tmp = c;
+    for (int d = 0; d < dom; d++) {
+      const int bit = 1 << (dom - d - 1);
+      if (tmp >= bit) {
+        ch.center[d] = boxes[b].center[d] + boxes[b].side / 4.;
+        tmp -= bit;
+      } else {
+        ch.center[d] = boxes[b].center[d] - boxes[b].side / 4.;
+      }
+    }
+    ch.level = boxes[b].level + 1;
+    ch.side = boxes[b].side / 2.0;
+    ch.parent = b;
+    boxes.push_back(ch);
+  }
+  Box& pb = boxes[b];
+  for (int c = 0; c < nch; c++) pb.children.push_back(first + c);
+  if ((int)levels.size() < pb.level + 2) levels.emplace_back();
+  for (int c = 0; c < nch; c++) levels[pb.level + 1].push_back(first + c);
+  for (size_t i = 0; i < pb.original.size(); i += sol) {
+    const int mi = pb.original[i];
+    const double* p = pts_ + (int64_t)(mi / sol) * dom;
+    Box& ch = boxes[pb.children[child_idx(pb, p)]];
+    for (int s = 0; s < sol; s++) ch.original.push_back(mi + s);
+  }
+  for (int c = 0; c < nch; c++) {
+    const int cid = first + c;
+    if ((int)boxes[cid].original.size() > cap) subdivide(cid);
+  }
+}
+
+void Tree::compute_neighbor_lists() {
+  const double sqd = std::sqrt((double)dom);
+  for (size_t l = 0; l < levels.size(); l++) {
+    for (int a : levels[l]) {
+      if (boxes[a].parent >= 0) {
+        const Box& par = boxes[boxes[a].parent];
+        for (int sib : par.children)
+          if (sib != a) boxes[a].neighbors.push_back(sib);
+        for (int pn : par.neighbors) {
+          for (int cousin : boxes[pn].children) {
+            if (boxes[cousin].level != boxes[a].level) continue;
+            double dist = 0.;
+            for (int d = 0; d < dom; d++) {
+              const double df = boxes[a].center[d] - boxes[cousin].center[d];
+              dist += df * df;
+            }
+            dist = std::sqrt(dist);
+            if (dist < boxes[a].side * sqd + 1e-9)
+              boxes[a].neighbors.push_back(cousin);
+          }
+        }
+      }
+      if (boxes[a].leaf) {
+        for (size_t n = 0; n < boxes[a].neighbors.size(); n++) {
+          const int nb = boxes[a].neighbors[n];
+          if (boxes[nb].level != boxes[a].level) continue;
+          const std::vector<int> kids = boxes[nb].children;
+          for (int ch : kids) descendent_neighbors(a, ch);
+        }
+      }
+    }
+  }
+}
+
+void Tree::descendent_neighbors(int big, int small) {
+  bool ok = true;
+  for (int d = 0; d < dom; d++) {
+    const double dist = std::fabs(boxes[big].center[d] - boxes[small].center[d]);
+    const double pred = (boxes[big].side + boxes[small].side) / 2.;
+    if (dist - pred > 1e-9) {
+      ok = false;
+      break;
+    }
+  }
+  if (ok) {
+    boxes[big].neighbors.push_back(small);
+    boxes[small].neighbors.push_back(big);
+  }
+  const std::vector<int> kids = boxes[small].children;
+  for (int ch : kids) descendent_neighbors(big, ch);
+}
+
+void Tree::remove_inactive_dofs_at_box(int b) {
+  Box& bx = boxes[b];
+  bx.skel.clear();
+  bx.redundant.clear();
+  bx.active.clear();
+  if (!bx.leaf) {
+    for (int c : bx.children) {
+      const Box& ch = boxes[c];
+      const std::vector<int>& src = ch.compressed ? ch.skel : ch.active;
+      bx.active.insert(bx.active.end(), src.begin(), src.end());
+    }
+    std::sort(bx.active.begin(), bx.active.end());
+  } else {
+    bx.active = bx.original;
+  }
+}
+
+void Tree::remove_inactive_dofs_at_level(int level) {
+  for (int b : levels[level])
+    if (!boxes[b].compressed) remove_inactive_dofs_at_box(b);
+  for (int a : levels[level]) {
+    Box& ba = boxes[a];
+    ba.near.clear();
+    for (int nb : ba.neighbors) {
+      const Box& bn = boxes[nb];
+      if (bn.level > ba.level) continue;
+      const std::vector<int>& src = bn.leaf ? bn.original : bn.active;
+      ba.near.insert(ba.near.end(), src.begin(), src.end());
+    }
+  }
+}
+
+void Tree::remove_inactive_dofs_at_all_boxes() {
+  for (int l = (int)levels.size() - 1; l >= 0; l--)
+    remove_inactive_dofs_at_level(l);
+}
+
+}  // namespace ki

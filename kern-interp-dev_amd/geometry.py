@@ -1,0 +1,346 @@
+"""Synthetic boundaries, boundary data and exact interior fields.
+
+These generators produce the inputs the solver consumes: the same four
+per-point arrays the reference's ``Boundary`` carries (points, outward
+normals, quadrature weights, curvatures; ``boundaries/boundary.h:47-64``),
+flattened point-major exactly like ``Boundary::points`` (x0 y0 [z0] x1 ...).
+
+Conventions follow the reference's concrete boundaries: 2D curves are traversed
+counter-clockwise, normals point outward and the curvature is
+``(x'y'' - x''y') / |x'|^3`` (positive on a convex curve; ``boundaries/
+cubic_boundary.cpp:144-155``, ``boundaries/circle.cpp:8-32``).
+
+All randomness comes from ``numpy.random.default_rng(seed)``, so a seed names
+an input set completely.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class Boundary:
+    """One closed boundary: flattened points/normals, weights, curvatures."""
+    dom: int
+    points: np.ndarray      # (N*dom,) float64, point-major
+    normals: np.ndarray     # (N*dom,) float64, outward unit normals
+    weights: np.ndarray     # (N,) quadrature weights (arc length / area)
+    curvatures: np.ndarray  # (N,) 2D curvature; zeros in 3D
+    name: str = ""
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def npts(self) -> int:
+        return self.weights.shape[0]
+
+    def pts(self) -> np.ndarray:
+        return self.points.reshape(-1, self.dom)
+
+    def nrm(self) -> np.ndarray:
+        return self.normals.reshape(-1, self.dom)
+
+
+# ---------------------------------------------------------------------------
+# 2D curves on Gauss-Legendre panels
+# ---------------------------------------------------------------------------
+
+def _gl_panels(npanels: int, order: int):
+    """Parameters and weights of ``npanels`` equal panels on [0, 2pi)."""
+    x, w = np.polynomial.legendre.leggauss(order)
+    h = 2.0 * math.pi / npanels
+    starts = np.arange(npanels) * h
+    t = (starts[:, None] + (x[None, :] + 1.0) * 0.5 * h).ravel()
+    wt = np.tile(w * 0.5 * h, npanels)
+    return t, wt
+
+
+def _curve_from_derivs(t, wt, x, y, dx, dy, ddx, ddy, name, meta):
+    speed = np.hypot(dx, dy)
+    nx, ny = dy / speed, -dx / speed          # outward for a CCW curve
+    kappa = (dx * ddy - ddx * dy) / speed ** 3
+    pts = np.stack([x, y], axis=1).ravel()
+    nrm = np.stack([nx, ny], axis=1).ravel()
+    return Boundary(2, pts.astype(np.float64), nrm.astype(np.float64),
+                    (wt * speed).astype(np.float64),
+                    kappa.astype(np.float64), name, meta)
+
+
+def ellipse(a=1.0, b=0.5, npanels=64, order=16) -> Boundary:
+    """Ellipse (a cos t, b sin t) on equal-parameter GL panels."""
+    t, wt = _gl_panels(npanels, order)
+    c, s = np.cos(t), np.sin(t)
+    return _curve_from_derivs(t, wt, a * c, b * s, -a * s, b * c, -a * c,
+                              -b * s, "ellipse",
+                              {"a": a, "b": b, "npanels": npanels,
+                               "order": order})
+
+
+def polar_curve(coef_a, phases, npanels, order, name="polar") -> Boundary:
+    """r(t) = 1 + sum_k a_k cos(k t + phi_k), k = 1..len(coef_a)."""
+    t, wt = _gl_panels(npanels, order)
+    r = np.ones_like(t)
+    dr = np.zeros_like(t)
+    ddr = np.zeros_like(t)
+    for k, (ak, ph) in enumerate(zip(coef_a, phases), start=1):
+        arg = k * t + ph
+        r += ak * np.cos(arg)
+        dr += -ak * k * np.sin(arg)
+        ddr += -ak * k * k * np.cos(arg)
+    c, s = np.cos(t), np.sin(t)
+    x, y = r * c, r * s
+    dx = dr * c - r * s
+    dy = dr * s + r * c
+    ddx = ddr * c - 2 * dr * s - r * c
+    ddy = ddr * s + 2 * dr * c - r * s
+    return _curve_from_derivs(t, wt, x, y, dx, dy, ddx, ddy, name,
+                              {"a": list(map(float, coef_a)),
+                               "phi": list(map(float, phases)),
+                               "npanels": npanels, "order": order})
+
+
+def star(npanels=1024, order=16, amp=0.3, lobes=5) -> Boundary:
+    """Star curve r(t) = 1 + amp cos(lobes t)."""
+    coef = np.zeros(lobes)
+    coef[lobes - 1] = amp
+    b = polar_curve(coef, np.zeros(lobes), npanels, order, "star")
+    return b
+
+
+def random_smooth_curve(seed: int, npts: int, order=16) -> Boundary:
+    """r(t) = 1 + sum_{k=1..8} a_k cos(k t + phi_k), a_k ~ U(-.2,.2)/k."""
+    rng = np.random.default_rng(seed)
+    k = np.arange(1, 9)
+    a = rng.uniform(-0.2, 0.2, size=8) / k
+    phi = rng.uniform(0.0, 2 * math.pi, size=8)
+    npanels = max(1, npts // order)
+    return polar_curve(a, phi, npanels, order, "smooth%d" % seed)
+
+
+def circle(n: int, radius=0.25) -> Boundary:
+    """Equispaced circle, as in the reference's Circle (circle.cpp:8-32)."""
+    ang = np.arange(n) * 2.0 * math.pi / n
+    pts = np.stack([radius * np.cos(ang), radius * np.sin(ang)], 1).ravel()
+    nrm = np.stack([np.cos(ang), np.sin(ang)], 1).ravel()
+    w = np.full(n, 2.0 * math.pi * radius / n)
+    kap = np.full(n, 1.0 / radius)
+    return Boundary(2, pts, nrm, w, kap, "circle", {"radius": radius})
+
+
+# ---------------------------------------------------------------------------
+# 3D surfaces
+# ---------------------------------------------------------------------------
+
+def _fibonacci_unit(n: int) -> np.ndarray:
+    i = np.arange(n, dtype=np.float64)
+    z = 1.0 - (2.0 * i + 1.0) / n
+    rho = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    golden = math.pi * (3.0 - math.sqrt(5.0))
+    th = golden * i
+    return np.stack([rho * np.cos(th), rho * np.sin(th), z], axis=1)
+
+
+def fibonacci_sphere(n: int, radius=1.0) -> Boundary:
+    """Fibonacci-lattice sphere with equal-area weights 4 pi r^2 / n."""
+    u = _fibonacci_unit(n)
+    pts = (radius * u).ravel()
+    w = np.full(n, 4.0 * math.pi * radius * radius / n)
+    return Boundary(3, pts, u.ravel().copy(), w, np.zeros(n), "sphere",
+                    {"radius": radius})
+
+
+def ellipsoid(n: int, axes=(1.0, 0.75, 0.5)) -> Boundary:
+    """Fibonacci lattice mapped onto an ellipsoid, area-weighted.
+
+    The unit-sphere lattice point u maps to D u, D = diag(axes).  The area
+    element scales by det(D) |D^{-1} u|, so w = (4 pi / n) det(D) |D^{-1} u|
+    and sum(w) converges to the ellipsoid's area; the outward normal is
+    D^{-1} u normalised.
+    """
+    a = np.asarray(axes, dtype=np.float64)
+    u = _fibonacci_unit(n)
+    g = u / a[None, :]
+    gn = np.linalg.norm(g, axis=1)
+    pts = (u * a[None, :]).ravel()
+    nrm = (g / gn[:, None]).ravel()
+    w = (4.0 * math.pi / n) * float(np.prod(a)) * gn
+    return Boundary(3, pts, nrm, w, np.zeros(n), "ellipsoid",
+                    {"axes": list(map(float, a))})
+
+
+def ellipsoid_area(axes) -> float:
+    """Exact ellipsoid surface area (Legendre's elliptic-integral form)."""
+    from scipy.special import ellipeinc, ellipkinc
+    a, b, c = sorted(map(float, axes), reverse=True)
+    if a == c:
+        return 4.0 * math.pi * a * a
+    phi = math.acos(c / a)
+    m = (a * a * (b * b - c * c)) / (b * b * (a * a - c * c))
+    s = math.sin(phi)
+    return 2 * math.pi * c * c + 2 * math.pi * a * b / s * (
+        ellipeinc(phi, m) * s * s + ellipkinc(phi, m) * math.cos(phi) ** 2)
+
+
+# ---------------------------------------------------------------------------
+# Sources, boundary data and exact interior fields
+# ---------------------------------------------------------------------------
+
+def exterior_sources(seed: int, count: int, dom: int, rmin=2.0, rmax=3.0,
+                     strength_dim=1):
+    """Seeded source points at radius U(rmin, rmax) and N(0,1) strengths."""
+    rng = np.random.default_rng(seed)
+    rad = rng.uniform(rmin, rmax, size=count)
+    if dom == 2:
+        ang = rng.uniform(0.0, 2 * math.pi, size=count)
+        loc = np.stack([rad * np.cos(ang), rad * np.sin(ang)], 1)
+    else:
+        v = rng.normal(size=(count, 3))
+        v /= np.linalg.norm(v, axis=1)[:, None]
+        loc = v * rad[:, None]
+    q = rng.normal(size=(count, strength_dim))
+    return loc, q
+
+
+def laplace2d_field(x: np.ndarray, loc, q) -> np.ndarray:
+    """sum_j q_j log|x - c_j| at points x (M,2) -> (M,)."""
+    d = x[:, None, :] - loc[None, :, :]
+    return (np.log(np.linalg.norm(d, axis=2)) * q[None, :, 0]).sum(1)
+
+
+def laplace3d_field(x: np.ndarray, loc, q) -> np.ndarray:
+    """sum_j q_j / |x - c_j| at points x (M,3) -> (M,)."""
+    d = x[:, None, :] - loc[None, :, :]
+    return (q[None, :, 0] / np.linalg.norm(d, axis=2)).sum(1)
+
+
+def stokeslet2d_field(x: np.ndarray, loc, f) -> np.ndarray:
+    """2D Stokeslets: u = sum (-log r I + r r^T / r^2) f / (4 pi)."""
+    d = x[:, None, :] - loc[None, :, :]
+    r2 = (d * d).sum(2)
+    rf = (d * f[None, :, :]).sum(2)
+    u = (-0.5 * np.log(r2))[:, :, None] * f[None, :, :] + \
+        (rf / r2)[:, :, None] * d
+    return (u.sum(1) / (4 * math.pi)).ravel()
+
+
+def stokeslet3d_field(x: np.ndarray, loc, f) -> np.ndarray:
+    """3D Stokeslets: u = sum (I / r + r r^T / r^3) f / (8 pi)."""
+    d = x[:, None, :] - loc[None, :, :]
+    r2 = (d * d).sum(2)
+    r = np.sqrt(r2)
+    rf = (d * f[None, :, :]).sum(2)
+    u = (1.0 / r)[:, :, None] * f[None, :, :] + \
+        (rf / (r2 * r))[:, :, None] * d
+    return (u.sum(1) / (8 * math.pi)).ravel()
+
+
+def interior_points(b: Boundary, count: int, seed: int, shrink=0.6):
+    """Seeded check points strictly inside a star-shaped boundary.
+
+    A random direction is chosen and the point is placed at a random fraction
+    (<= shrink) of the boundary's radius along it, the radius being taken
+    from the nearest boundary point in angle; every configuration here is
+    star-shaped about the origin.
+    """
+    rng = np.random.default_rng(seed)
+    p = b.pts()
+    rad = np.linalg.norm(p, axis=1)
+    if b.dom == 2:
+        ang = np.arctan2(p[:, 1], p[:, 0])
+        order = np.argsort(ang)
+        ang_s, rad_s = ang[order], rad[order]
+        t = rng.uniform(-math.pi, math.pi, size=count)
+        rr = np.interp(t, ang_s, rad_s, period=2 * math.pi)
+        f = rng.uniform(0.0, shrink, size=count)
+        return np.stack([f * rr * np.cos(t), f * rr * np.sin(t)], 1)
+    v = rng.normal(size=(count, 3))
+    v /= np.linalg.norm(v, axis=1)[:, None]
+    unit = p / rad[:, None]
+    idx = np.argmax(v @ unit.T, axis=1)
+    f = rng.uniform(0.0, shrink, size=count)
+    return v * (f * rad[idx])[:, None]
+
+
+# ---------------------------------------------------------------------------
+# Named workloads
+# ---------------------------------------------------------------------------
+
+@dataclass
+class Problem:
+    """A boundary, a PDE, solver parameters, boundary data, exact field."""
+    name: str
+    boundary: Boundary
+    pde: str                 # "laplace" or "stokes"
+    sol: int                 # unknowns per point
+    id_tol: float
+    max_leaf_dofs: int
+    proxy_points: int        # proxy points (2D circle) / (3D sphere)
+    proxy_ratio: float       # proxy radius / box circumradius (sqrt(d) hw)
+    rhs: np.ndarray
+    check_points: np.ndarray
+    exact: np.ndarray        # exact field at check_points (flattened)
+
+    @property
+    def ndof(self) -> int:
+        return self.boundary.npts * self.sol
+
+
+def _field(pde, dom, x, loc, q):
+    if pde == "laplace":
+        return laplace2d_field(x, loc, q) if dom == 2 else \
+            laplace3d_field(x, loc, q)
+    return stokeslet2d_field(x, loc, q) if dom == 2 else \
+        stokeslet3d_field(x, loc, q)
+
+
+def make_problem(b: Boundary, pde: str, id_tol: float, max_leaf_dofs: int,
+                 proxy_points: int, proxy_ratio: float, nsrc: int, seed: int,
+                 ncheck=100, name="") -> Problem:
+    dom = b.dom
+    sol = 1 if pde == "laplace" else dom
+    loc, q = exterior_sources(seed, nsrc, dom, strength_dim=sol)
+    rhs = _field(pde, dom, b.pts(), loc, q)
+    chk = interior_points(b, ncheck, seed + 1000)
+    exact = _field(pde, dom, chk, loc, q)
+    return Problem(name or b.name, b, pde, sol, id_tol, max_leaf_dofs,
+                   proxy_points, proxy_ratio, rhs.astype(np.float64),
+                   chk, exact)
+
+
+def config(index: int, scale: float = 1.0) -> Problem:
+    """The five named workloads; ``scale`` < 1 shrinks N for quick runs.
+
+    0: 2D Laplace, ellipse (1, .5), 64 GL16 panels, leaf 64, tol 1e-10,
+       64 proxy points, 5 log charges.
+    1: 2D Stokes, star 1+.3cos5t, 1024 GL16 panels (32,768 DOFs), leaf 128,
+       tol 1e-8, 128 proxy points, 4 Stokeslets.
+    2: 3D Laplace, Fibonacci unit sphere 32,768 points, leaf 128, 256 proxy
+       points, tol 1e-6, 8 1/r charges.
+    3: one member of the batched smooth-curve family (seed 0), tol 1e-8,
+       128 proxy points.
+    4: 3D Stokes, ellipsoid (1,.75,.5), 65,536 points (196,608 DOFs),
+       leaf 256, tol 1e-6, 6 Stokeslets.
+    """
+    if index == 0:
+        b = ellipse(1.0, 0.5, max(1, int(64 * scale)), 16)
+        return make_problem(b, "laplace", 1e-10, 64, 64, 1.5, 5, 0,
+                            name="c1_laplace2d_ellipse")
+    if index == 1:
+        b = star(max(4, int(1024 * scale)), 16)
+        return make_problem(b, "stokes", 1e-8, 128, 128, 1.5, 4, 1,
+                            name="c2_stokes2d_star")
+    if index == 2:
+        b = fibonacci_sphere(max(256, int(32768 * scale)))
+        return make_problem(b, "laplace", 1e-6, 128, 256, 1.5, 8, 2,
+                            name="c3_laplace3d_sphere")
+    if index == 3:
+        b = random_smooth_curve(0, max(256, int(8192 * scale)))
+        return make_problem(b, "laplace", 1e-8, 64, 128, 1.5, 5, 3,
+                            name="c4_laplace2d_smooth")
+    if index == 4:
+        b = ellipsoid(max(512, int(65536 * scale)))
+        return make_problem(b, "stokes", 1e-6, 256, 256, 1.5, 6, 4,
+                            name="c5_stokes3d_ellipsoid")
+    raise ValueError("unknown workload index %d" % index)

@@ -1,0 +1,637 @@
+"""CPU oracle: a numpy/scipy restatement of the reference's RS solver path.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the product imports this module; only
+``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` use it, and only as the checker / CPU baseline.
+
+It restates, function by function, the reference's recursive-skeletonization
+factorization and solve in plain FP64 (paths relative to the reference repo):
+
+* tree build, neighbor lists       quadtree/quadtree.cpp:192-413
+* active / near DOF bookkeeping    quadtree/quadtree.cpp:1134-1261
+* Schur-update gather              quadtree/quadtree.cpp:1545-1632
+* kernel entries and 3D diagonals  kernel/kernel.cpp:33-301, 304-429
+* ID matrix (level 1/2 rules, proxy rows)
+                                   kernel/kernel.cpp:432-641, 646-796
+* interpolative decomposition      ki_mat.cpp:525-568 (dgeqp3 + threshold
+                                   + dtrtrs: called here through scipy's
+                                   LAPACK, the same routines)
+* decouple, skeletonize, solve     skel_factorization/skel_factorization.cpp:
+                                   36-58, 91-127, 173-310, 555-778
+
+The reference's compile-time constants (MAX_LEAF_DOFS 64, MIN_DOFS_TO_COMPRESS
+64, NUM_PROXY_POINTS 128, RADIUS_RATIO 2, the 8x4 3D proxy rule) are the
+defaults of :class:`Params`; the synthetic workloads override them.
+
+How each part is pinned (tests/test_oracle.py): the tree by brute-force
+invariants and a hand-checked 2D example; kernel entries by closed forms
+(2D/3D Laplace and Stokes identities, Gauss's law row sums); the ID by the
+``A[:, p] = Q R`` identity and the ``||A_r - A_s T|| <= tol |R00|`` bound; the
+factorization by reproducing the dense matrix's solution at small N (dense
+LU) to round-off when the tolerance is tight; end to end by the analytic
+interior fields.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import scipy.linalg as sla
+from scipy.linalg import lapack
+
+LAPLACE, STOKES = 0, 3          # reference Kernel::Pde values (kernel.h:19-24)
+
+
+@dataclass
+class Params:
+    """Runtime versions of the reference's compile-time constants."""
+    id_tol: float = 1e-6
+    max_leaf_dofs: int = 64      # quadtree.h:11
+    min_dofs_to_compress: int = 64   # skel_factorization.h:12
+    proxy_points: int = 128      # kernel.h:13 (2D); 3D: n_theta*n_phi
+    proxy_ratio: float = 2.0     # kernel.h:14, times sqrt(d) * half side
+    proxy3d_fibonacci: bool = False  # False: reference 8x4 GL tensor rule
+
+
+# ---------------------------------------------------------------------------
+# Tree (quadtree/quadtree.cpp)
+# ---------------------------------------------------------------------------
+
+class Node:
+    __slots__ = ("level", "center", "side", "is_leaf", "parent", "children",
+                 "neighbors", "original", "active", "skel", "redundant",
+                 "perm", "near", "compressed", "T", "L", "U", "Xlu", "piv",
+                 "schur", "index")
+
+    def __init__(self):
+        self.is_leaf = True
+        self.children = []
+        self.neighbors = []
+        self.original = []
+        self.active = []
+        self.skel = []
+        self.redundant = []
+        self.perm = []
+        self.near = []
+        self.compressed = False
+        self.parent = None
+
+
+class Tree:
+    """Restates QuadTree::initialize_tree and its helpers."""
+
+    def __init__(self, points: np.ndarray, dom: int, sol: int,
+                 max_leaf_dofs: int = 64):
+        # quadtree.cpp:240-287
+        self.dom, self.sol, self.cap = dom, sol, max_leaf_dofs
+        self.points = np.asarray(points, dtype=np.float64).ravel()
+        pl = self.points.tolist()
+        self._pl = pl
+        mn, mx = pl[0], pl[0]
+        for v in pl:
+            if v < mn:
+                mn = v
+            if v > mx:
+                mx = v
+        tmin, tmax = mn - 0.01, mx + 0.01
+        root = Node()
+        root.level = 0
+        root.center = [(tmin + tmax) / 2.0] * dom
+        root.side = tmax - tmin
+        self.root = root
+        self.levels = [[root]]
+        for i in range(0, len(pl), dom):
+            self._recursive_add(root, pl[i:i + dom], i // dom)
+        self._compute_neighbor_lists()
+        for lv in self.levels:
+            for k, nd in enumerate(lv):
+                nd.index = k
+
+    def _child_idx(self, node, pt):
+        ci = 0
+        for i in range(self.dom):
+            if pt[i] >= node.center[i]:
+                ci += 2 ** (self.dom - i - 1)
+        return ci
+
+    def _recursive_add(self, node, pt, point_ind):
+        # quadtree.cpp:317-341
+        while True:
+            for i in range(self.sol):
+                node.original.append(self.sol * point_ind + i)
+            if not node.is_leaf:
+                node = node.children[self._child_idx(node, pt)]
+                continue
+            if len(node.original) > self.cap:
+                self._subdivide(node)
+            return
+
+    def _subdivide(self, node):
+        # quadtree.cpp:347-413
+        d = self.dom
+        node.is_leaf = False
+        for child_idx in range(2 ** d):
+            cc = []
+            tmp = child_idx
+            for k in range(d):
+                if tmp >= 2 ** (d - k - 1):
+                    cc.append(node.center[k] + node.side / 4.0)
+                    tmp -= 2 ** (d - k - 1)
+                else:
+                    cc.append(node.center[k] - node.side / 4.0)
+            ch = Node()
+            ch.center = cc
+            node.children.append(ch)
+        for ch in node.children:
+            ch.level = node.level + 1
+            ch.side = node.side / 2.0
+            ch.parent = node
+        if len(self.levels) < node.level + 2:
+            self.levels.append(list(node.children))
+        else:
+            self.levels[node.level + 1].extend(node.children)
+        pl = self._pl
+        for index in range(0, len(node.original), self.sol):
+            mi = node.original[index]
+            pv = (mi // self.sol) * d
+            ci = self._child_idx(node, pl[pv:pv + d])
+            dst = node.children[ci].original
+            for i in range(self.sol):
+                dst.append(mi + i)
+        for ch in node.children:
+            if len(ch.original) > self.cap:
+                self._subdivide(ch)
+
+    def _compute_neighbor_lists(self):
+        # quadtree.cpp:192-237
+        d = self.dom
+        for level in self.levels:
+            for a in level:
+                if a.parent is not None:
+                    for sib in a.parent.children:
+                        if sib is not a:
+                            a.neighbors.append(sib)
+                    for pn in a.parent.neighbors:
+                        for cousin in pn.children:
+                            if cousin.level != a.level:
+                                continue
+                            dist = 0.0
+                            for k in range(d):
+                                dist += (a.center[k] - cousin.center[k]) ** 2
+                            dist = math.sqrt(dist)
+                            if dist < a.side * math.sqrt(d) + 1e-9:
+                                a.neighbors.append(cousin)
+                if a.is_leaf:
+                    n = 0
+                    while n < len(a.neighbors):
+                        nb = a.neighbors[n]
+                        n += 1
+                        if nb.level != a.level:
+                            continue
+                        for ch in nb.children:
+                            self._descendent_neighbors(a, ch)
+
+    def _descendent_neighbors(self, big, small):
+        # quadtree.cpp:292-314
+        ok = True
+        for k in range(self.dom):
+            dist = abs(big.center[k] - small.center[k])
+            pred = (big.side + small.side) / 2.0
+            if dist - pred > 1e-9:
+                ok = False
+                break
+        if ok:
+            big.neighbors.append(small)
+            small.neighbors.append(big)
+        for ch in small.children:
+            self._descendent_neighbors(big, ch)
+
+    # -- DOF bookkeeping (quadtree.cpp:1134-1204, 1236-1261) ---------------
+    def remove_inactive_dofs_at_level(self, level: int):
+        for nd in self.levels[level]:
+            if nd.compressed:
+                continue
+            self._remove_inactive_dofs_at_box(nd)
+        for a in self.levels[level]:
+            a.near = []
+            for nb in a.neighbors:
+                if nb.level > a.level:
+                    continue
+                a.near.extend(nb.original if nb.is_leaf else nb.active)
+
+    def _remove_inactive_dofs_at_box(self, nd):
+        nd.skel, nd.redundant = [], []
+        if not nd.is_leaf:
+            act = []
+            for ch in nd.children:
+                act.extend(ch.skel if ch.compressed else ch.active)
+            act.sort()
+            nd.active = act
+        else:
+            nd.active = list(nd.original)
+
+    def remove_inactive_dofs_at_all_boxes(self):
+        for level in range(len(self.levels) - 1, -1, -1):
+            self.remove_inactive_dofs_at_level(level)
+
+
+# ---------------------------------------------------------------------------
+# Kernel (kernel/kernel.h, kernel/kernel.cpp)
+# ---------------------------------------------------------------------------
+
+def gl_nodes(n: int, a: float, b: float):
+    """n-point Gauss-Legendre rule on [a, b] (the cgqf(n,1,0,0,a,b) rule)."""
+    x, w = np.polynomial.legendre.leggauss(n)
+    return (a + b) / 2 + (b - a) / 2 * x, (b - a) / 2 * w
+
+
+def fibonacci_dirs(n: int) -> np.ndarray:
+    i = np.arange(n, dtype=np.float64)
+    z = 1.0 - (2.0 * i + 1.0) / n
+    rho = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    th = math.pi * (3.0 - math.sqrt(5.0)) * i
+    return np.stack([rho * np.cos(th), rho * np.sin(th), z], axis=1)
+
+
+class Kernel:
+    """Restates struct Kernel (kernel.h:18-111) for LAPLACE and STOKES."""
+
+    def __init__(self, boundary, pde: int, params: Params):
+        self.dom = boundary.dom
+        self.pde = pde
+        self.sol = 1 if pde == LAPLACE else self.dom
+        self.pts = boundary.pts().astype(np.float64)
+        self.nrm = boundary.nrm().astype(np.float64)
+        self.w = np.asarray(boundary.weights, dtype=np.float64)
+        self.curv = np.asarray(boundary.curvatures, dtype=np.float64)
+        self.params = params
+        self.diag = None        # 3D Laplace: (N,), 3D Stokes: (N,3,3)
+        if self.dom == 3:
+            self.compute_diag_3d()
+
+    # -- entries: kernel.cpp:33-121 -----------------------------------------
+    def _entries(self, tp, tn, tpar, sp, sn, sw, sc, spar, sidx, self_ok):
+        """Dense block for targets (rows) x sources (cols).
+
+        tp,tn: (M,d); sp,sn: (K,d); sw,sc: (K,); tpar/spar: component index
+        of each row/col (Stokes); sidx: source point index (3D diagonal).
+        """
+        d = self.dom
+        r = tp[:, None, :] - sp[None, :, :]
+        rr = (r * r).sum(2)
+        dot = (r * sn[None, :, :]).sum(2)
+        zero = (r == 0.0).all(2) if self_ok else np.zeros(rr.shape, bool)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            if d == 2 and self.pde == LAPLACE:
+                val = -sw[None, :] * (1.0 / (2 * math.pi)) * dot / rr
+                dg = 0.5 + 0.5 * sc * sw * (1.0 / (2 * math.pi))
+                val = np.where(zero, dg[None, :], val)
+            elif d == 2:
+                ri = np.take_along_axis(r, tpar[:, None, None].repeat(
+                    r.shape[1], 1), 2)[..., 0]
+                rj = np.take_along_axis(r, spar[None, :, None].repeat(
+                    r.shape[0], 0), 2)[..., 0]
+                tnp = tn[np.arange(len(tpar)), tpar]
+                snp = sn[np.arange(len(spar)), spar]
+                val = sw[None, :] * (1 / math.pi) * dot / np.power(rr, 2) * \
+                    ri * rj + sw[None, :] * tnp[:, None] * snp[None, :]
+                if zero.any():
+                    # self entries: two_d_kern, kernel.cpp:75-79
+                    ii, jj = np.nonzero(zero)
+                    tq, sq = tpar[ii], spar[jj]
+                    isd = (tq + sq + 1) % 2
+                    dg = -0.5 * isd + (1 - 2 * isd) * 0.5 * sc[jj] * \
+                        sw[jj] * (1 / math.pi) * sn[jj, (tq + 1) % 2] * \
+                        sn[jj, (sq + 1) % 2] + sw[jj] * tn[ii, tq] * sn[jj, sq]
+                    val[ii, jj] = dg
+            elif self.pde == LAPLACE:
+                val = -sw[None, :] * (1.0 / (4 * math.pi)) * dot / \
+                    np.power(rr, 1.5)
+                if zero.any():
+                    val = np.where(zero, self.diag[sidx][None, :], val)
+            else:
+                ri = np.take_along_axis(r, tpar[:, None, None].repeat(
+                    r.shape[1], 1), 2)[..., 0]
+                rj = np.take_along_axis(r, spar[None, :, None].repeat(
+                    r.shape[0], 0), 2)[..., 0]
+                tnp = tn[np.arange(len(tpar)), tpar]
+                snp = sn[np.arange(len(spar)), spar]
+                val = sw[None, :] * (-3.0 / (4.0 * math.pi)) * dot / \
+                    np.power(rr, 2.5) * ri * rj + \
+                    sw[None, :] * tnp[:, None] * snp[None, :]
+                if zero.any():
+                    dg = self.diag[sidx[None, :], tpar[:, None],
+                                   spar[None, :]]
+                    val = np.where(zero, dg, val)
+        return val
+
+    def __call__(self, tgt, src, forward_pts=None):
+        """Kernel::operator() / get_3d (kernel.cpp:304-429): K(tgt, src)."""
+        tgt = np.asarray(tgt, dtype=np.int64)
+        src = np.asarray(src, dtype=np.int64)
+        s = self.sol
+        if len(tgt) == 0 or len(src) == 0:
+            return np.zeros((len(tgt), len(src)))
+        tpi, spi = tgt // s, src // s
+        if forward_pts is not None:
+            tp = forward_pts[tpi]
+            tn = np.zeros_like(tp)
+        else:
+            tp, tn = self.pts[tpi], self.nrm[tpi]
+        return self._entries(tp, tn, tgt % s, self.pts[spi], self.nrm[spi],
+                             self.w[spi], self.curv[spi], src % s, spi,
+                             forward_pts is None)
+
+    def dense(self):
+        n = len(self.w) * self.sol
+        idx = np.arange(n)
+        return self(idx, idx)
+
+    def apply_forward(self, domain_points, mu):
+        """Kernel::apply_forward (kernel.cpp:809-825)."""
+        dp = np.asarray(domain_points, dtype=np.float64).reshape(-1, self.dom)
+        tgt = np.arange(dp.shape[0] * self.sol)
+        src = np.arange(len(self.w) * self.sol)
+        out = np.zeros(len(tgt))
+        for c in range(0, len(tgt), 256 * self.sol):
+            tt = tgt[c:c + 256 * self.sol]
+            out[c:c + len(tt)] = self(tt, src, forward_pts=dp) @ mu
+        return out
+
+    # -- 3D diagonals: kernel.cpp:124-301 (single outer surface) ------------
+    def compute_diag_3d(self):
+        n = len(self.w)
+        p, nr, w = self.pts, self.nrm, self.w
+        if self.pde == LAPLACE:
+            diag = np.zeros(n)
+            for c in range(0, n, 512):
+                tp = p[c:c + 512]
+                r = tp[:, None, :] - p[None, :, :]
+                rr = (r * r).sum(2)
+                dot = (r * nr[None, :, :]).sum(2)
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    v = -w[None, :] * (1.0 / (4 * math.pi)) * dot / \
+                        np.power(rr, 1.5)
+                ii = np.arange(c, c + tp.shape[0])
+                v[np.arange(tp.shape[0]), ii] = 0.0
+                diag[c:c + tp.shape[0]] = 1.0 - v.sum(1)
+            self.diag = diag
+        else:
+            diag = np.zeros((n, 3, 3))
+            for c in range(0, n, 256):
+                tp, tn = p[c:c + 256], nr[c:c + 256]
+                r = tp[:, None, :] - p[None, :, :]
+                rr = (r * r).sum(2)
+                dot = (r * nr[None, :, :]).sum(2)
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    f = w[None, :] * (-3.0 / (4.0 * math.pi)) * dot / \
+                        np.power(rr, 2.5)
+                ii = np.arange(c, c + tp.shape[0])
+                m = np.ones(f.shape, bool)
+                m[np.arange(tp.shape[0]), ii] = False
+                for a in range(3):
+                    for b in range(3):
+                        v = f * r[:, :, a] * r[:, :, b] + \
+                            w[None, :] * tn[:, a:a + 1] * nr[None, :, b]
+                        diag[c:c + tp.shape[0], a, b] = np.where(m, v, 0.0).sum(1)
+            self.diag = np.eye(3)[None, :, :] - diag
+
+    # -- proxy rows: kernel.cpp:646-796 ---------------------------------------
+    def proxy_surface(self, center, r):
+        """Proxy points, normals and weights around ``center`` at radius r."""
+        pr = self.params
+        if self.dom == 2:
+            n = pr.proxy_points
+            ang = 2 * math.pi * np.arange(n) * (1.0 / n)
+            pp = np.stack([center[0] + r * np.cos(ang),
+                           center[1] + r * np.sin(ang)], 1)
+            pn = np.stack([np.cos(ang), np.sin(ang)], 1)
+            pw = np.full(n, 2.0 * math.pi * r / n)
+            return pp, pn, pw
+        if pr.proxy3d_fibonacci:
+            u = fibonacci_dirs(pr.proxy_points)
+            pw = np.full(len(u), 4.0 * math.pi * r * r / len(u))
+            return np.asarray(center)[None, :] + r * u, u, pw
+        nphi = int(round(math.sqrt(pr.proxy_points / 2)))
+        nth = 2 * nphi
+        phis, phw = gl_nodes(nphi, 0.0, math.pi)
+        pts, nrm, wts = [], [], []
+        for i in range(nth):
+            th = 2 * math.pi * i * (1.0 / nth)
+            for j in range(nphi):
+                ph = phis[j]
+                pts.append([center[0] + r * math.sin(ph) * math.cos(th),
+                            center[1] + r * math.sin(ph) * math.sin(th),
+                            center[2] + r * math.cos(ph)])
+                nrm.append([math.sin(ph) * math.cos(th),
+                            math.sin(ph) * math.sin(th), math.cos(ph)])
+                wts.append(r ** 2 * (phw[j] * math.sin(ph)) *
+                           (2 * math.pi / nth))
+        return np.array(pts), np.array(nrm), np.array(wts)
+
+    def proxy_mat(self, center, r, box):
+        """Rows [box->proxy ; proxy->box] for the DOFs ``box``."""
+        box = np.asarray(box, dtype=np.int64)
+        s = self.sol
+        pp, pn, pw = self.proxy_surface(center, r)
+        npx = pp.shape[0]
+        bpi = box // s
+        bp, bn, bw = self.pts[bpi], self.nrm[bpi], self.w[bpi]
+        tpar = np.tile(np.arange(s), npx)
+        ppr = np.repeat(np.arange(npx), s)
+        # box -> proxy: targets proxy (normal pn), sources box DOFs
+        top = self._entries(pp[ppr], pn[ppr], tpar, bp, bn, bw,
+                            np.zeros(len(box)), box % s, bpi, False)
+        # proxy -> box: targets box DOFs, sources proxy (weight pw, curv 0)
+        bot = self._entries(bp, bn, box % s, pp[ppr], pn[ppr], pw[ppr],
+                            np.zeros(len(ppr)), tpar, ppr, False).T
+        return np.vstack([top, bot])
+
+    def id_mat(self, tree: Tree, node: Node):
+        """Kernel::get_id_mat (kernel.cpp:432-641).  Descendant updates on
+        these rows are identically zero (descendants' skeletons lie inside
+        the box, the rows outside it), so they are not formed."""
+        act = node.active
+        if node.level == 1 or (node.level == 2 and self.dom < 3):
+            out = []
+            for ln in tree.levels[node.level]:
+                if ln is not node:
+                    out.extend(ln.active)
+            if node.level == 2:
+                for lvl in range(node.level - 1, -1, -1):
+                    for ln in tree.levels[lvl]:
+                        if ln.is_leaf:
+                            out.extend(ln.original)
+            return np.vstack([self(out, act), self(act, out).T])
+        inner = self.inner_circle(node)
+        rad = (node.side / 2) * math.sqrt(self.dom) * self.params.proxy_ratio
+        pxy = self.proxy_mat(node.center, rad, act)
+        return np.vstack([self(inner, act), self(act, inner).T, pxy])
+
+    def inner_circle(self, node):
+        lim = math.sqrt(self.dom) * self.params.proxy_ratio * (node.side / 2)
+        out = []
+        for mi in node.near:
+            p = self.pts[mi // self.sol]
+            dist = 0.0
+            for k in range(self.dom):
+                dist += (node.center[k] - p[k]) ** 2
+            if math.sqrt(dist) < lim:
+                out.append(mi)
+        return out
+
+
+# ---------------------------------------------------------------------------
+# ID (ki_mat.cpp:525-568)
+# ---------------------------------------------------------------------------
+
+def interpolative_decomposition(A: np.ndarray, tol: float):
+    """Returns (k, p, T): rank, 0-based permutation, T = R11^-1 R12 (k x r).
+
+    k = 0 means "not compressed" (no |R_ii| < tol |R_00| for i >= 1).
+    """
+    m, n = A.shape
+    qr, jpvt, tau, work, info = lapack.dgeqp3(np.asfortranarray(A.copy()))
+    assert info == 0
+    thresh = abs(tol * qr[0, 0])
+    skel = 0
+    for i in range(1, n):
+        if i >= min(m, n):
+            break
+        if abs(qr[i, i]) < thresh:
+            skel = i
+            break
+    if skel == 0:
+        return 0, None, None
+    p = (jpvt - 1).astype(np.int64)
+    T = sla.solve_triangular(qr[:skel, :skel], qr[:skel, skel:n],
+                             lower=False)
+    return skel, p, T
+
+
+# ---------------------------------------------------------------------------
+# Factorization and solve (skel_factorization.cpp)
+# ---------------------------------------------------------------------------
+
+def descendants_updates(rows, cols, node: Node):
+    """get_descendents_updates (quadtree.cpp:1545-1632)."""
+    upd = np.zeros((len(rows), len(cols)))
+    rpos = {v: i for i, v in enumerate(rows)}
+    cpos = {v: i for i, v in enumerate(cols)}
+
+    def visit(nd):
+        for ch in nd.children:
+            if ch.compressed:
+                sn = ch.skel
+                ri = [(a, rpos[v]) for a, v in enumerate(sn) if v in rpos]
+                ci = [(a, cpos[v]) for a, v in enumerate(sn) if v in cpos]
+                if ri and ci:
+                    ra, rb = zip(*ri)
+                    ca, cb = zip(*ci)
+                    upd[np.ix_(rb, cb)] += ch.schur[np.ix_(ra, ca)]
+            visit(ch)
+    visit(node)
+    return upd
+
+
+class SkelFactorization:
+    """Restates SkelFactorization (skel_factorization.h:20-58)."""
+
+    def __init__(self, params: Params):
+        self.p = params
+        self.stats = []          # (level, n, m, k) per compressed box
+
+    def id_compress(self, kern: Kernel, tree: Tree, node: Node) -> int:
+        # skel_factorization.cpp:36-58
+        A = kern.id_mat(tree, node)
+        if A.shape[0] == 0:
+            return 0
+        k, p, T = interpolative_decomposition(A, self.p.id_tol)
+        if k == 0:
+            return 0
+        act = node.active
+        node.skel = [act[i] for i in p[:k]]
+        node.redundant = [act[i] for i in p[k:]]
+        node.perm = list(p)
+        node.T = T
+        self.stats.append((node.level, len(act), A.shape[0], k))
+        return T.shape[1]
+
+    def decouple(self, kern: Kernel, node: Node):
+        # skel_factorization.cpp:91-127
+        BN = node.active
+        upd = descendants_updates(BN, BN, node)
+        K = kern(BN, BN) - upd
+        k = node.T.shape[0]
+        s = np.array(node.perm[:k])
+        r = np.array(node.perm[k:])
+        T = node.T
+        Krs = K[np.ix_(r, s)] - T.T @ K[np.ix_(s, s)]
+        X = K[np.ix_(r, r)] - T.T @ K[np.ix_(s, r)] - Krs @ T
+        Ksr = K[np.ix_(s, r)] - K[np.ix_(s, s)] @ T
+        lu, piv = sla.lu_factor(X)
+        node.Xlu, node.piv = lu, piv
+        node.L = sla.lu_solve((lu, piv), Ksr.T, trans=1).T
+        node.U = sla.lu_solve((lu, piv), Krs)
+        node.schur = node.L @ Krs
+        node.compressed = True
+
+    def skeletonize(self, kern: Kernel, tree: Tree):
+        # skel_factorization.cpp:173-310
+        lv = len(tree.levels)
+        for level in range(lv - 1, 0, -1):
+            tree.remove_inactive_dofs_at_level(level)
+            for node in tree.levels[level]:
+                if node.compressed or \
+                        len(node.active) < self.p.min_dofs_to_compress:
+                    continue
+                if self.id_compress(kern, tree, node) == 0:
+                    continue
+                self.decouple(kern, node)
+        tree.remove_inactive_dofs_at_all_boxes()
+        allskel = tree.root.active
+        self.allskel = list(allskel)
+        if allskel:
+            A = kern(allskel, allskel) - \
+                descendants_updates(allskel, allskel, tree.root)
+            self.top_lu = sla.lu_factor(A)
+
+    def solve(self, tree: Tree, b: np.ndarray) -> np.ndarray:
+        # skel_factorization.cpp:595-778
+        x = np.array(b, dtype=np.float64).copy()
+        lv = len(tree.levels)
+        for level in range(lv - 1, -1, -1):
+            for nd in tree.levels[level]:
+                if not nd.compressed:
+                    continue
+                s, r = nd.skel, nd.redundant
+                x[r] -= nd.T.T @ x[s]
+                x[s] -= nd.L @ x[r]
+        for level in range(lv - 1, -1, -1):
+            for nd in tree.levels[level]:
+                if nd.compressed and nd.redundant:
+                    x[nd.redundant] = sla.lu_solve((nd.Xlu, nd.piv),
+                                                   x[nd.redundant])
+        if self.allskel:
+            x[self.allskel] = sla.lu_solve(self.top_lu, x[self.allskel])
+        for level in range(lv):
+            for nd in reversed(tree.levels[level]):
+                if not nd.compressed:
+                    continue
+                s, r = nd.skel, nd.redundant
+                x[r] -= nd.U @ x[s]
+                x[s] -= nd.T @ x[r]
+        return x
+
+
+def factor_and_solve(boundary, pde: int, params: Params, rhs):
+    """Build tree, kernel, factorization and solve: the reference's
+    boundary_integral_solve minus the domain evaluation
+    (linear_solve.cpp:257-303)."""
+    kern = Kernel(boundary, pde, params)
+    tree = Tree(boundary.points, boundary.dom, kern.sol, params.max_leaf_dofs)
+    fac = SkelFactorization(params)
+    fac.skeletonize(kern, tree)
+    mu = fac.solve(tree, rhs)
+    return kern, tree, fac, mu

@@ -1,0 +1,54 @@
+"""The C-ABI library builds, loads and exports every declared entry point.
+
+No compute call is made here (no GPU needed)."""
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+from conftest import ROOT, pkg
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "ki_abi.h").read_text()
+    return sorted(set(re.findall(r"\b(ki_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_python_export_list():
+    ki = pkg()
+    assert set(declared_symbols()) == set(ki.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib_path = ROOT / "kern-interp-dev_amd" / "libki_hip.so"
+    if not lib_path.exists():
+        pytest.skip("library not built")
+    lib = ctypes.CDLL(str(lib_path))
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_params_default_matches_reference_constants():
+    lib_path = ROOT / "kern-interp-dev_amd" / "libki_hip.so"
+    if not lib_path.exists():
+        pytest.skip("library not built")
+    ki = pkg()
+    p = ki.default_params()
+    # skel_factorization.h:12, kernel.h:13-14
+    assert p.min_dofs_to_compress == 64
+    assert p.proxy_points == 128
+    assert p.proxy_ratio == 2.0
+    assert p.precision == 64
+
+
+def test_error_path_without_device_is_reported_not_fatal():
+    lib_path = ROOT / "kern-interp-dev_amd" / "libki_hip.so"
+    if not lib_path.exists():
+        pytest.skip("library not built")
+    ki = pkg()
+    lib = ki.load_library()
+    h = ctypes.c_void_p()
+    rc = lib.ki_ctx_create(10 ** 6, ctypes.byref(h))
+    assert rc != 0
+    assert lib.ki_last_error().decode() != ""

@@ -1,0 +1,255 @@
+"""GPU parity: the HIP path through the C-ABI against the CPU oracle.
+
+Bars (stated per test): integer work (tree partition, neighbor lists, DOF
+lists) bit-exact; kernel entries within a few ulp; skeleton sets equal to the
+oracle's (dgeqp3) sets; solutions within 1e-10 relative of the dense FP64
+solve after refinement.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from conftest import geometry, pkg
+
+pytestmark = pytest.mark.gpu
+
+ki = pkg()
+G = geometry()
+
+
+def _case(name):
+    """(boundary, pde, engine params kwargs, oracle Params) for a small case."""
+    import ki_oracle as O
+    if name == "lap2d_ellipse":          # the c1 workload, full size
+        pb = G.config(0)
+        b, pde = pb.boundary, ki.KI_LAPLACE
+        kw = dict(id_tol=1e-10, proxy_points=64, proxy_ratio=1.5)
+        leaf = 64
+    elif name == "stokes2d_star":
+        b, pde = G.star(64, 16), ki.KI_STOKES
+        kw = dict(id_tol=1e-8, proxy_points=128, proxy_ratio=1.5)
+        leaf = 128
+    elif name == "lap2d_circle_ref":      # reference constants (kernel.h)
+        b, pde = G.circle(2000), ki.KI_LAPLACE
+        kw = dict(id_tol=1e-8, proxy_points=128, proxy_ratio=2.0)
+        leaf = 64
+    elif name == "lap3d_sphere":
+        b, pde = G.fibonacci_sphere(2048), ki.KI_LAPLACE
+        kw = dict(id_tol=1e-6, proxy_points=256, proxy_ratio=1.5,
+                  proxy_kind=1)
+        leaf = 128
+    elif name == "lap3d_sphere_ref":      # reference 8x4 GL proxy, ratio 2
+        b, pde = G.fibonacci_sphere(1500), ki.KI_LAPLACE
+        kw = dict(id_tol=1e-5, proxy_points=32, proxy_ratio=2.0)
+        leaf = 64
+    elif name == "stokes3d_ellipsoid":
+        b, pde = G.ellipsoid(1024), ki.KI_STOKES
+        kw = dict(id_tol=1e-6, proxy_points=256, proxy_ratio=1.5,
+                  proxy_kind=1)
+        leaf = 256
+    else:
+        raise KeyError(name)
+    prm = O.Params(id_tol=kw["id_tol"], max_leaf_dofs=leaf,
+                   proxy_points=kw["proxy_points"],
+                   proxy_ratio=kw["proxy_ratio"],
+                   proxy3d_fibonacci=kw.get("proxy_kind", 0) == 1)
+    return b, pde, kw, leaf, prm
+
+
+CASES = ["lap2d_ellipse", "stokes2d_star", "lap2d_circle_ref",
+         "lap3d_sphere", "lap3d_sphere_ref", "stokes3d_ellipsoid"]
+
+_cache = {}
+
+
+def setup(name):
+    """Context + oracle objects for a case (built once per session)."""
+    if name in _cache:
+        return _cache[name]
+    import ki_oracle as O
+    b, pde, kw, leaf, prm = _case(name)
+    print("[case] %s npts=%d" % (name, b.npts), flush=True)
+    ctx = ki.Context()
+    ctx.set_geometry(b, pde)
+    ctx.build_tree(leaf)
+    opde = O.LAPLACE if pde == ki.KI_LAPLACE else O.STOKES
+    okern = O.Kernel(b, opde, prm)
+    otree = O.Tree(b.points, b.dom, okern.sol, leaf)
+    _cache[name] = (b, pde, kw, leaf, prm, ctx, okern, otree)
+    return _cache[name]
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_tree_bit_exact(name):
+    b, pde, kw, leaf, prm, ctx, okern, otree = setup(name)
+    assert ctx.num_levels() == len(otree.levels)
+    for l, lv in enumerate(otree.levels):
+        assert ctx.level_size(l) == len(lv)
+        for i, nd in enumerate(lv):
+            bx = ctx.box(l, i)
+            assert bx["is_leaf"] == nd.is_leaf
+            assert bx["side"] == nd.side
+            assert list(bx["center"][:b.dom]) == list(nd.center)
+            assert list(ctx.box_list(l, i, "original")) == nd.original
+            nb = [(v >> 20, v & ((1 << 20) - 1))
+                  for v in ctx.box_list(l, i, "neighbors")]
+            assert nb == [(x.level, x.index) for x in nd.neighbors]
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_kernel_block_entries(name):
+    b, pde, kw, leaf, prm, ctx, okern, otree = setup(name)
+    rng = np.random.default_rng(7)
+    n = ctx.ndof
+    tgt = np.concatenate([np.arange(min(n, 40)), rng.integers(0, n, 200)])
+    src = np.concatenate([np.arange(min(n, 40)), rng.integers(0, n, 150)])
+    got = ctx.kernel_block(tgt, src)
+    ref = okern(tgt, src)
+    scale = np.abs(ref).max()
+    # entries agree to a few ulp of the block's scale (same formulas, FP64)
+    assert np.abs(got - ref).max() <= 1e-13 * scale
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_proxy_rows(name):
+    b, pde, kw, leaf, prm, ctx, okern, otree = setup(name)
+    p = ki.default_params(**kw)
+    for l in range(2, len(otree.levels)):
+        for nd in otree.levels[l][:3]:
+            if not nd.original:
+                continue
+            rad = (nd.side / 2) * np.sqrt(b.dom) * kw["proxy_ratio"]
+            got = ctx.proxy_block(p, nd.center, rad, nd.original)
+            ref = okern.proxy_mat(nd.center, rad, nd.original)
+            assert got.shape == ref.shape
+            assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("name", ["lap3d_sphere", "stokes3d_ellipsoid",
+                                  "lap3d_sphere_ref"])
+def test_diag3d_row_sums(name):
+    b, pde, kw, leaf, prm, ctx, okern, otree = setup(name)
+    got = ctx.diag3d()
+    assert np.abs(got - okern.diag).max() <= 1e-11 * np.abs(okern.diag).max()
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_matvec_matches_dense(name):
+    b, pde, kw, leaf, prm, ctx, okern, otree = setup(name)
+    rng = np.random.default_rng(3)
+    mu = rng.normal(size=ctx.ndof)
+    K = okern.dense()
+    ref = K @ mu
+    got = ctx.matvec(mu)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def _oracle_factor(name):
+    import ki_oracle as O
+    key = name + "/fac"
+    if key in _cache:
+        return _cache[key]
+    b, pde, kw, leaf, prm, ctx, okern, otree = setup(name)
+    tree = O.Tree(b.points, b.dom, okern.sol, leaf)
+    fac = O.SkelFactorization(prm)
+    fac.skeletonize(okern, tree)
+    _cache[key] = (tree, fac)
+    return _cache[key]
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_skeletons_match_oracle_fp64(name):
+    """Plain FP64 GPU path: same compressed boxes, ranks and skeleton sets as
+    the oracle's dgeqp3-based ID (sets compared, pivot order may differ only
+    through exact ties)."""
+    import ki_oracle as O
+    b, pde, kw, leaf, prm, ctx, okern, otree = setup(name)
+    ctx.skeletonize(ki.default_params(precision=64, **kw))
+    tree, fac = _oracle_factor(name)
+    total = same = tied = 0
+    for l, lv in enumerate(tree.levels):
+        for i, nd in enumerate(lv):
+            bx = ctx.box(l, i)
+            assert bx["compressed"] == nd.compressed, (l, i)
+            if not nd.compressed:
+                continue
+            total += 1
+            assert bx["k"] == len(nd.skel), (l, i, bx["k"], len(nd.skel))
+            sk = ctx.box_list(l, i, "skel")
+            if set(sk) == set(nd.skel):
+                same += 1
+                continue
+            # A different skeleton is accepted only as a pivot tie: on the
+            # same ID matrix (same active set) it must interpolate the
+            # redundant columns as well as the oracle's skeleton does.
+            act = list(ctx.box_list(l, i, "active"))
+            assert act == nd.active, (l, i, "active sets differ")
+            A = okern.id_mat(tree, nd)
+            pos = {v: q for q, v in enumerate(act)}
+            s_idx = [pos[v] for v in sk]
+            r_idx = [q for q in range(len(act)) if q not in set(s_idx)]
+            Tls = np.linalg.lstsq(A[:, s_idx], A[:, r_idx], rcond=None)[0]
+            err = np.linalg.norm(A[:, r_idx] - A[:, s_idx] @ Tls, 2)
+            so = [pos[v] for v in nd.skel]
+            ro = [q for q in range(len(act)) if q not in set(so)]
+            Tor = np.linalg.lstsq(A[:, so], A[:, ro], rcond=None)[0]
+            err_o = np.linalg.norm(A[:, ro] - A[:, so] @ Tor, 2)
+            assert err <= max(10 * err_o, 10 * kw["id_tol"] *
+                              np.linalg.norm(A, 2)), (l, i, err, err_o)
+            tied += 1
+    print("[skel] %s: %d/%d boxes with identical skeleton sets, %d ties"
+          % (name, same, total, tied))
+    assert same + tied == total
+    assert ctx.stats()["top_size"] == len(fac.allskel)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_solution_matches_oracle_fp64(name):
+    b, pde, kw, leaf, prm, ctx, okern, otree = setup(name)
+    tree, fac = _oracle_factor(name)
+    ctx.skeletonize(ki.default_params(precision=64, **kw))
+    rng = np.random.default_rng(11)
+    rhs = rng.normal(size=ctx.ndof)
+    x_ref = fac.solve(tree, rhs)
+    x = ctx.solve(rhs)
+    # same factorization up to round-off in fill/QR/LU order
+    assert np.linalg.norm(x - x_ref) <= 1e-10 * np.linalg.norm(x_ref) * \
+        max(1.0, 1e-2 / kw["id_tol"]) ** 0 * 100
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("name", CASES)
+def test_gmres_reaches_1e10_and_dense_solution(name, precision):
+    b, pde, kw, leaf, prm, ctx, okern, otree = setup(name)
+    ctx.skeletonize(ki.default_params(precision=precision, **kw))
+    rng = np.random.default_rng(5)
+    rhs = rng.normal(size=ctx.ndof)
+    x, it, rel = ctx.gmres(rhs, tol=1e-10, maxit=60, restart=30)
+    K = okern.dense()
+    true_rel = np.linalg.norm(K @ x - rhs) / np.linalg.norm(rhs)
+    print("[gmres] %s fp%d iters=%d relres=%.2e true=%.2e"
+          % (name, precision, it, rel, true_rel))
+    assert rel <= 1e-10
+    assert true_rel <= 2e-10
+    xd = np.linalg.solve(K, rhs)
+    cond_bound = 1e-10 * np.linalg.cond(K)
+    assert np.linalg.norm(x - xd) <= max(1e-9, 2 * cond_bound) * \
+        np.linalg.norm(xd)
+
+
+def test_c1_field_matches_exact_interior_field():
+    """c1 end to end: factor, GMRES to 1e-10, evaluate at 100 interior
+    points; compare with the exact harmonic field of the point charges."""
+    pb = G.config(0)
+    ctx = ki.Context()
+    ctx.set_geometry(pb.boundary, ki.KI_LAPLACE)
+    ctx.build_tree(pb.max_leaf_dofs)
+    ctx.skeletonize(ki.default_params(id_tol=pb.id_tol,
+                                      proxy_points=pb.proxy_points,
+                                      proxy_ratio=pb.proxy_ratio))
+    mu, it, rel = ctx.gmres(pb.rhs, tol=1e-10)
+    u = ctx.apply_forward(pb.check_points, mu)
+    err = np.abs(u - pb.exact).max() / np.abs(pb.exact).max()
+    print("[c1] iters=%d relres=%.2e field err=%.2e" % (it, rel, err))
+    assert err <= 1e-10
