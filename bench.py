@@ -1,0 +1,211 @@
+"""Throughput benchmark: factor + solve to relative residual 1e-10.
+
+Workload (default): 3D Stokes double layer on the ellipsoid (1, .75, .5) with
+65,536 area-weighted Fibonacci points (196,608 DOFs), octree leaf <= 256
+DOFs, ID tolerance 1e-6, 256-point Fibonacci proxy sphere at 1.5x the box
+circumradius, right-hand side from 6 seeded exterior Stokeslets.
+
+A step is one full pass of the solve path: skeletonize (factor) + FP64 GMRES
+preconditioned by the factorization until ||K mu - f|| / ||f|| <= 1e-10.
+The measured form is FP32 fill / ID / Schur with FP64 top LU, residual and
+GMRES; the plain FP64 GPU factorization of the same workload is timed beside
+it.  Inputs (geometry, diagonals, right-hand side) are resident on the GPU
+before the timed region starts.
+
+Prints one JSON line (rank 0).  Single GPU only.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = ("factor+solve s and DOF/s at 1e-10 residual: FP32+learned vs "
+          "plain FP64 GPU")
+PEAK_HBM = 8.0e12          # B/s, MI355X HBM3E
+PEAK_FP32 = 157.3e12       # FLOP/s dense FP32 (vector = matrix on MI355X)
+PEAK_FP64 = 78.6e12
+
+
+def params_for(ki, pb, precision, predictor=0):
+    return ki.default_params(id_tol=pb.id_tol, proxy_points=pb.proxy_points,
+                             proxy_ratio=pb.proxy_ratio,
+                             proxy_kind=1 if pb.boundary.dom == 3 else 0,
+                             precision=precision, use_predictor=predictor)
+
+
+def one_step(ctx, prm, tol, maxit, restart):
+    t0 = time.perf_counter()
+    ctx.skeletonize(prm)
+    t1 = time.perf_counter()
+    it, rel = ctx.gmres_resident(tol=tol, maxit=maxit, restart=restart)
+    t2 = time.perf_counter()
+    return t1 - t0, t2 - t1, it, rel
+
+
+def cpu_baseline(G, index, budget_s=20.0):
+    """The oracle (numpy/scipy restatement, FP64) on a bounded sample: the
+    same workload family shrunk until one factor + solve fits the budget."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import ki_oracle as O
+    try:
+        import threadpoolctl
+        cores = threadpoolctl.threadpool_info()[0].get("num_threads", 1)
+    except Exception:  # pragma: no cover
+        cores = os.cpu_count() or 1
+    scale = 1.0 / 64
+    while True:
+        pb = G.config(index, scale)
+        b = pb.boundary
+        prm = O.Params(id_tol=pb.id_tol, max_leaf_dofs=pb.max_leaf_dofs,
+                       proxy_points=pb.proxy_points,
+                       proxy_ratio=pb.proxy_ratio,
+                       proxy3d_fibonacci=b.dom == 3)
+        pde = O.LAPLACE if pb.pde == "laplace" else O.STOKES
+        t0 = time.perf_counter()
+        kern, tree, fac, mu = O.factor_and_solve(b, pde, prm, pb.rhs)
+        dt = time.perf_counter() - t0
+        if dt > budget_s / 4 or scale >= 1.0:
+            break
+        scale *= 2
+    return {"value": pb.ndof / dt, "unit": "DOF/s", "cores": cores,
+            "kind": "port",
+            "sample": "oracle factor+solve of the same workload family at "
+                      "%d points (%d DOFs), %.1f s, one direct solve at ID "
+                      "tol %.0e (no refinement)" % (b.npts, pb.ndof, dt,
+                                                   pb.id_tol)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=4)
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--tol", type=float, default=1e-10)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: the solver's multi-GPU "
+              "sharding is a design, not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    ki = importlib.import_module("kern-interp-dev_amd")
+    G = importlib.import_module("kern-interp-dev_amd.geometry")
+    pb = G.config(args.config, args.scale)
+    b = pb.boundary
+    pde = ki.KI_LAPLACE if pb.pde == "laplace" else ki.KI_STOKES
+    print("[bench] %s: %d points, %d DOFs" % (pb.name, b.npts, pb.ndof),
+          file=sys.stderr, flush=True)
+    ctx = ki.Context(0)
+    t0 = time.perf_counter()
+    ctx.set_geometry(b, pde)      # uploads + 3D row-sum diagonals
+    t_setup = time.perf_counter() - t0
+    ctx.build_tree(pb.max_leaf_dofs)
+    ctx.set_rhs(pb.rhs)
+    maxit, restart = 60, 30
+    prm32 = params_for(ki, pb, 32)
+    for w in range(args.warmup):
+        tf, ts, it, rel = one_step(ctx, prm32, args.tol, maxit, restart)
+        print("[bench] warmup %d: factor %.3f s solve %.3f s iters %d "
+              "relres %.2e" % (w, tf, ts, it, rel), file=sys.stderr,
+              flush=True)
+    times = []
+    stats = []
+    for k in range(args.steps):
+        tf, ts, it, rel = one_step(ctx, prm32, args.tol, maxit, restart)
+        st = ctx.stats()
+        times.append((tf, ts, it, rel))
+        stats.append(st)
+        print("[bench] step %d: factor %.3f s solve %.3f s iters %d relres "
+              "%.2e" % (k, tf, ts, it, rel), file=sys.stderr, flush=True)
+    mu32 = ctx.solution()
+    tot = [a + b2 for a, b2, _, _ in times]
+    ms = 1e3 * float(np.mean(tot))
+    value = pb.ndof / (ms * 1e-3)
+    worst_rel = max(t[3] for t in times)
+
+    # plain FP64 GPU factorization of the same workload, same refinement
+    fp64 = None
+    if not args.no_fp64:
+        prm64 = params_for(ki, pb, 64)
+        tf, ts, it, rel = one_step(ctx, prm64, args.tol, maxit, restart)
+        tf, ts, it, rel = one_step(ctx, prm64, args.tol, maxit, restart)
+        mu64 = ctx.solution()
+        fp64 = {"factor_s": tf, "solve_s": ts, "iters": it, "relres": rel,
+                "dof_per_s": pb.ndof / (tf + ts),
+                "speedup_fp32_vs_fp64": (tf + ts) / (ms * 1e-3),
+                "rel_diff_vs_fp64": float(np.linalg.norm(mu32 - mu64) /
+                                          np.linalg.norm(mu64))}
+
+    # roofline of the dominant kernel (phase) in the timed region
+    st = stats[-1]
+    phases = {"cpqr": (st["t_qr"], st["bytes_qr"], "hbm"),
+              "fill": (st["t_fill"], st["bytes_fill"], "hbm"),
+              "decouple": (st["t_decouple"], st["flops_schur"], "mfma")}
+    dom = max(phases, key=lambda k: phases[k][0])
+    t_d, work, bound = phases[dom]
+    if bound == "hbm":
+        achieved = work / t_d
+        roof = {"bound": "hbm", "achieved": achieved / 1e9,
+                "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                "frac": achieved / PEAK_HBM, "traffic": None,
+                "kernel": dom}
+    else:
+        achieved = work / t_d
+        roof = {"bound": "mfma", "achieved": achieved / 1e12,
+                "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP32, "traffic": None,
+                "kernel": dom}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(G, args.config)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "DOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64",
+        "data": "synthetic",
+        "config": {"workload": pb.name, "points": b.npts, "dofs": pb.ndof,
+                   "leaf_dofs": pb.max_leaf_dofs, "id_tol": pb.id_tol,
+                   "proxy_points": pb.proxy_points,
+                   "proxy_ratio": pb.proxy_ratio, "residual_tol": args.tol,
+                   "parallelism": "single-gpu"},
+        "factor_s": float(np.mean([t[0] for t in times])),
+        "solve_s": float(np.mean([t[1] for t in times])),
+        "gmres_iters": times[-1][2],
+        "relres": worst_rel,
+        "setup_s": t_setup,
+        "top_size": st["top_size"],
+        "phase_s": {"fill": st["t_fill"], "cpqr": st["t_qr"],
+                    "decouple": st["t_decouple"], "top": st["t_top"]},
+        "plain_fp64": fp64,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -25,7 +25,9 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <string>
 #include <unordered_map>
@@ -261,8 +263,29 @@ void skeletonize_t(Ctx& c) {
   const int P = 2 * sol * px.npts;
   const double t_start = now();
   std::vector<int> pos(c.ndof, -1);  // DOF -> position scratch
+  const bool verbose = std::getenv("KI_VERBOSE") != nullptr;
 
   for (int level = L - 1; level >= 1; level--) {
+    // per-level log line (KI_VERBOSE), printed however the level ends
+    const double lv_t0 = now();
+    const ki_stats st0 = st;
+    int lv_nc = 0, lv_ncomp = 0, lv_maxn = 0, lv_maxm = 0;
+    int64_t lv_sumn = 0, lv_summ = 0, lv_sumk = 0;
+    struct LevelLog {
+      std::function<void()> f;
+      ~LevelLog() { f(); }
+    } lvlog{[&] {
+      if (!verbose) return;
+      std::fprintf(stderr,
+                   "[ki] level %d: id %d comp %d sum_n %lld max_n %d "
+                   "sum_m %lld max_m %d sum_k %lld | fill %.4f qr %.4f "
+                   "dec %.4f wall %.4f s\n",
+                   level, lv_nc, lv_ncomp, (long long)lv_sumn, lv_maxn,
+                   (long long)lv_summ, lv_maxm, (long long)lv_sumk,
+                   st.t_fill - st0.t_fill, st.t_qr - st0.t_qr,
+                   st.t_decouple - st0.t_decouple, now() - lv_t0);
+      std::fflush(stderr);
+    }};
     tr.remove_inactive_dofs_at_level(level);
     // ---- ID matrices -------------------------------------------------
     std::vector<int> cand, idx;
@@ -324,6 +347,13 @@ void skeletonize_t(Ctx& c) {
       cand.push_back(b);
     }
     const int nc = cand.size();
+    lv_nc = nc;
+    for (const QrDesc& q : qd) {
+      lv_sumn += q.n;
+      lv_summ += q.m;
+      lv_maxn = std::max(lv_maxn, q.n);
+      lv_maxm = std::max(lv_maxm, q.m);
+    }
     std::vector<int> rank(nc, 0), hpiv(piv_el);
     if (nc > 0) {
       c.idm.ensure(sizeof(T) * idm_el);
@@ -380,6 +410,7 @@ void skeletonize_t(Ctx& c) {
       bx.compressed = true;
       comp.push_back(cand[q]);
       comp_q.push_back(q);
+      lv_sumk += k;
       c.boxlog.insert(c.boxlog.end(), {level, n, qd[q].m, k});
       st.boxes_compressed++;
     }
@@ -461,6 +492,7 @@ void skeletonize_t(Ctx& c) {
                            c.mapb.as<int>(), F, (T)1, (T*)nullptr, s);
       KI_HIP(hipStreamSynchronize(s));
     }
+    lv_ncomp = comp.size();
     if (comp.empty()) continue;
 
     // ---- decouple (skel_factorization.cpp:91-127) ----------------------

@@ -1,0 +1,386 @@
+"""Pins the CPU oracle (oracle/ki_oracle.py) without the reference binary.
+
+The reference holds no golden vectors for this path, so each oracle function
+is checked against what mathematics fixes: closed-form kernel values on the
+unit sphere, Gauss / stresslet identities, quadrature known answers, brute-
+force tree invariants and a hand-worked tree, LAPACK QR identities, and
+dense-LU solutions.  Each check is chosen so that a dropped term, a wrong
+sign, a transposed operand or a wrong index fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from conftest import geometry
+
+G = geometry()
+
+
+@pytest.fixture(scope="module")
+def O():
+    import ki_oracle
+    return ki_oracle
+
+
+# ---------------------------------------------------------------------------
+# Geometry known answers
+# ---------------------------------------------------------------------------
+
+def test_ellipse_weights_sum_to_perimeter():
+    from scipy.special import ellipe
+    b = G.ellipse(1.0, 0.5, 64, 16)
+    exact = 4 * 1.0 * ellipe(1 - 0.25)
+    assert abs(b.weights.sum() - exact) < 1e-12 * exact
+
+
+def test_ellipse_normals_curvature():
+    b = G.ellipse(1.0, 0.5, 8, 16)
+    p, n = b.pts(), b.nrm()
+    # outward unit normal of x^2/a^2 + y^2/b^2 = 1 is grad/|grad|
+    g = np.stack([p[:, 0] / 1.0, p[:, 1] / 0.25], 1)
+    g /= np.linalg.norm(g, axis=1)[:, None]
+    assert np.abs(n - g).max() < 1e-13
+    # curvature of the ellipse: ab / (a^2 sin^2 t + b^2 cos^2 t)^1.5
+    t = np.arctan2(p[:, 1] / 0.5, p[:, 0])
+    k = 0.5 / (np.sin(t) ** 2 + 0.25 * np.cos(t) ** 2) ** 1.5
+    assert np.abs(b.curvatures - k).max() < 1e-12
+
+
+def test_sphere_and_ellipsoid_areas():
+    s = G.fibonacci_sphere(4096)
+    assert abs(s.weights.sum() - 4 * math.pi) < 1e-12
+    e = G.ellipsoid(65536)
+    ex = G.ellipsoid_area((1, .75, .5))
+    assert abs(e.weights.sum() - ex) < 1e-6 * ex
+    n = e.nrm()
+    p = e.pts()
+    g = p / np.array([1, .75 ** 2, .25])[None, :]
+    g /= np.linalg.norm(g, axis=1)[:, None]
+    assert np.abs(n - g).max() < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# Tree
+# ---------------------------------------------------------------------------
+
+def test_tree_hand_example(O):
+    # 2D, one DOF per point, cap 2.  Points chosen so that the root splits,
+    # then only the upper-right quadrant splits again.
+    pts = np.array([[0.9, 0.9], [0.6, 0.6], [0.8, 0.55], [0.1, 0.1],
+                    [0.55, 0.95], [0.95, 0.6]])
+    t = O.Tree(pts.ravel(), 2, 1, 2)
+    # root box: all coordinates in [0.1, 0.95], padded by 0.01:
+    # [0.09, 0.96], centre 0.525, side 0.87
+    assert t.root.side == pytest.approx(0.87)
+    assert t.root.center == pytest.approx([0.525, 0.525])
+    # child index bit 2^(d-1-i): x >= c gives 2, y >= c gives 1
+    kids = t.root.children
+    assert [len(k.original) for k in kids] == [1, 0, 0, 5]
+    assert kids[0].original == [3]
+    assert kids[3].original == [0, 1, 2, 4, 5]
+    # quadrant 3 (centre 0.7425) splits when point 2 arrives:
+    # pt0 -> 3, pt1 -> 0, pt2 -> 2, later pt4 -> 1, pt5 -> 2
+    assert not kids[3].is_leaf and len(t.levels) == 3
+    g = kids[3].children
+    assert [c.original for c in g] == [[1], [4], [2, 5], [0]]
+    assert all(c.is_leaf for c in g)
+    # level order: root's children, then the split quadrant's children
+    assert t.levels[1] == kids
+    assert t.levels[2] == g
+
+
+def _leaves(t):
+    return [n for lv in t.levels for n in lv if n.is_leaf]
+
+
+@pytest.mark.parametrize("dom,n,sol,cap", [(2, 3000, 1, 64), (2, 1500, 2, 128),
+                                           (3, 3000, 1, 128), (3, 800, 3, 96)])
+def test_tree_invariants(O, dom, n, sol, cap):
+    b = G.star(n // 16, 16) if dom == 2 else G.ellipsoid(n)
+    t = O.Tree(b.points, dom, sol, cap)
+    ndof = b.npts * sol
+    seen = np.zeros(ndof, int)
+    for lf in _leaves(t):
+        assert len(lf.original) <= cap
+        for v in lf.original:
+            seen[v] += 1
+        assert lf.original == sorted(lf.original)
+    assert (seen == 1).all()
+    pts = b.pts()
+    for lv in t.levels:
+        for nd in lv:
+            if nd.is_leaf:
+                continue
+            assert len(nd.original) > cap
+            assert sorted(sum((c.original for c in nd.children), [])) == \
+                nd.original
+            for c in nd.children:
+                assert c.side == nd.side / 2
+                # every point of a child lies in the child's half-open box
+                for v in c.original[::sol]:
+                    p = pts[v // sol]
+                    for k in range(dom):
+                        lo = c.center[k] - c.side / 2
+                        hi = c.center[k] + c.side / 2
+                        assert lo - 1e-12 <= p[k] <= hi + 1e-12
+    # same-level neighbor lists equal brute-force adjacency
+    for lv in t.levels:
+        for a in lv:
+            same = {id(x) for x in a.neighbors if x.level == a.level}
+            brute = {id(x) for x in lv if x is not a and
+                     all(abs(x.center[k] - a.center[k]) <= a.side + 1e-9
+                         for k in range(dom))}
+            assert same == brute
+
+
+# ---------------------------------------------------------------------------
+# Kernel entries
+# ---------------------------------------------------------------------------
+
+def test_laplace2d_rows_sum_to_one(O):
+    # (1/2 + K) 1 = 1: the interior limit of the potential of a unit density
+    b = G.ellipse(1.0, 0.5, 64, 16)
+    k = O.Kernel(b, O.LAPLACE, O.Params())
+    K = k.dense()
+    assert np.abs(K.sum(1) - 1.0).max() < 1e-12
+
+
+def test_stokes2d_constant_density(O):
+    # stresslet identity: the double layer of a constant density c is -c
+    # inside, so the interior limit operator maps c to -c; the rank-one
+    # completion adds n_t (c . integral of n) = 0 on a closed curve.
+    b = G.star(64, 16)
+    k = O.Kernel(b, O.STOKES, O.Params())
+    K = k.dense()
+    for c in ([1.0, 0.0], [0.3, -0.7]):
+        mu = np.tile(c, b.npts)
+        assert np.abs(K @ mu + mu).max() < 1e-10
+
+
+def test_stokes2d_rank_one_completion(O):
+    # mu = normal field: integral of n.n ds = perimeter, so the completion
+    # term contributes w_j n_t (n_j . n_j) summed = perimeter * n_t.
+    b = G.ellipse(1.0, 0.5, 64, 16)
+    k = O.Kernel(b, O.STOKES, O.Params())
+    K = k.dense()
+    mu = b.normals.copy()
+    full = K @ mu
+    Ks = K.copy()
+    nn = np.outer(b.normals, b.normals)
+    w = np.repeat(b.weights, 2)
+    Ks -= nn * w[None, :]
+    per = b.weights.sum()
+    assert np.abs((full - Ks @ mu) - per * b.normals).max() < 1e-12
+
+
+def test_laplace3d_unit_sphere_closed_form(O):
+    # on the unit sphere (t - s).n_s = -|t - s|^2 / 2, so
+    # K_ij = -w/(4 pi) (r.n)/r^3 = w_j / (8 pi r_ij)
+    b = G.fibonacci_sphere(600)
+    k = O.Kernel(b, O.LAPLACE, O.Params())
+    K = k.dense()
+    p = b.pts()
+    r = np.linalg.norm(p[:, None, :] - p[None, :, :], axis=2)
+    np.fill_diagonal(r, 1.0)
+    ref = b.weights[None, :] / (8 * math.pi * r)
+    off = ~np.eye(len(r), dtype=bool)
+    assert np.abs(K[off] - ref[off]).max() < 1e-12 * ref.max()
+    # diagonal = 1 - row sum of the off-diagonal part (~1/2 on the sphere:
+    # the integral of 1/|t-s| over the unit sphere is 4 pi)
+    assert np.abs(np.diag(K) - (1 - (K * off).sum(1))).max() < 1e-13
+    assert np.abs(np.diag(K) - 0.5).max() < 0.05
+
+
+def test_stokes3d_unit_sphere_closed_form(O):
+    # K_ab = 3 w/(8 pi) r_a r_b / r^3 + w n_t,a n_s,b on the unit sphere
+    b = G.fibonacci_sphere(200)
+    k = O.Kernel(b, O.STOKES, O.Params())
+    K = k.dense()
+    p, n, w = b.pts(), b.nrm(), b.weights
+    i, j = 3, 17
+    r = p[i] - p[j]
+    rr = np.linalg.norm(r)
+    blk = 3 * w[j] / (8 * math.pi) * np.outer(r, r) / rr ** 3 + \
+        w[j] * np.outer(n[i], n[j])
+    assert np.abs(K[3 * i:3 * i + 3, 3 * j:3 * j + 3] - blk).max() < 1e-14
+    # the 3x3 diagonal blocks make each block row of K sum to the identity
+    S = K.reshape(len(w), 3, len(w), 3).sum(axis=2)
+    assert np.abs(S - np.eye(3)[None, :, :]).max() < 1e-12
+
+
+def test_forward_field_gauss_law_3d(O):
+    # -D[1] = 1 at interior points (Gauss), to quadrature accuracy
+    b = G.fibonacci_sphere(4000)
+    k = O.Kernel(b, O.LAPLACE, O.Params())
+    x = np.array([[0.1, -0.2, 0.05], [0.0, 0.3, -0.1]])
+    u = k.apply_forward(x, np.ones(b.npts))
+    assert np.abs(u - 1.0).max() < 1e-3
+
+
+# ---------------------------------------------------------------------------
+# Proxy rows
+# ---------------------------------------------------------------------------
+
+def test_proxy_rows_gauss_identity_2d(O):
+    # proxy -> box rows: potential of a unit density on the proxy circle at
+    # points inside it is 1 (trapezoid rule on a circle: spectral)
+    b = G.ellipse(1.0, 0.5, 16, 16)
+    prm = O.Params(proxy_points=128)
+    k = O.Kernel(b, O.LAPLACE, prm)
+    box = list(range(0, 40))
+    c = b.pts()[box].mean(0)
+    R = 1.5 * np.linalg.norm(b.pts()[box] - c, axis=1).max()
+    P = k.proxy_mat(c, R, box)
+    assert P.shape == (2 * 128, len(box))
+    assert np.abs(P[128:].sum(0) - 1.0).max() < 1e-10
+    # box -> proxy rows: the proxy points are outside the curve piece; the
+    # rows are the double layer of the box DOFs seen from the proxy points
+    pp, pn, pw = k.proxy_surface(c, R)
+    assert abs(pw.sum() - 2 * math.pi * R) < 1e-12
+
+
+@pytest.mark.parametrize("fib", [False, True])
+def test_proxy_rows_gauss_identity_3d(O, fib):
+    b = G.fibonacci_sphere(400)
+    prm = O.Params(proxy_points=32 if not fib else 256,
+                   proxy3d_fibonacci=fib)
+    k = O.Kernel(b, O.LAPLACE, prm)
+    box = list(range(0, 30))
+    c = b.pts()[box].mean(0)
+    R = 2.0 * np.linalg.norm(b.pts()[box] - c, axis=1).max()
+    P = k.proxy_mat(c, R, box)
+    npx = 32 if not fib else 256
+    assert P.shape == (2 * npx, len(box))
+    pp, pn, pw = k.proxy_surface(c, R)
+    # weights integrate the sphere area; the GL theta x phi rule is inexact
+    assert abs(pw.sum() - 4 * math.pi * R * R) < (1e-12 if fib else 0.01) * \
+        4 * math.pi * R * R
+    assert np.abs(P[npx:].sum(0) - 1.0).max() < 0.1
+
+
+def test_gl_rule_matches_textbook(O):
+    x, w = O.gl_nodes(4, 0.0, math.pi)
+    # 4-point Gauss-Legendre on [-1, 1]: +-0.3399810436, +-0.8611363116
+    ref = np.array([-0.8611363115940526, -0.3399810435848563,
+                    0.3399810435848563, 0.8611363115940526])
+    assert np.abs(x - math.pi / 2 * (ref + 1)).max() < 1e-14
+    assert abs(w.sum() - math.pi) < 1e-14
+
+
+# ---------------------------------------------------------------------------
+# Interpolative decomposition
+# ---------------------------------------------------------------------------
+
+def test_id_exact_low_rank(O):
+    rng = np.random.default_rng(0)
+    A = rng.normal(size=(120, 5)) @ rng.normal(size=(5, 40))
+    k, p, T = O.interpolative_decomposition(A, 1e-10)
+    assert k == 5
+    s, r = p[:k], p[k:]
+    assert np.abs(A[:, r] - A[:, s] @ T).max() < 1e-10 * np.abs(A).max()
+
+
+def test_id_rank_rule_and_pivots(O):
+    rng = np.random.default_rng(1)
+    U, _ = np.linalg.qr(rng.normal(size=(200, 60)))
+    V, _ = np.linalg.qr(rng.normal(size=(60, 60)))
+    sv = 10.0 ** -np.linspace(0, 12, 60)
+    A = U @ np.diag(sv) @ V.T
+    tol = 1e-6
+    k, p, T = O.interpolative_decomposition(A, tol)
+    q, r, pv = sla.qr(A, pivoting=True, mode="economic")
+    assert list(pv) == list(p)          # same LAPACK pivots
+    d = np.abs(np.diag(r))
+    assert d[k] < tol * d[0] <= d[k - 1]
+    assert (d[1:k] >= tol * d[0]).all()
+    s, rr = p[:k], p[k:]
+    err = np.linalg.norm(A[:, rr] - A[:, s] @ T, 2)
+    assert err <= 10 * tol * d[0] * math.sqrt(A.shape[1])
+
+
+def test_id_full_rank_returns_zero(O):
+    rng = np.random.default_rng(2)
+    A = rng.normal(size=(50, 20))
+    k, p, T = O.interpolative_decomposition(A, 1e-12)
+    assert k == 0 and p is None
+
+
+# ---------------------------------------------------------------------------
+# Factorization and solve
+# ---------------------------------------------------------------------------
+
+def test_factorization_matches_dense_solve_2d(O):
+    pb = G.config(0)
+    prm = O.Params(id_tol=1e-12, max_leaf_dofs=64, proxy_points=64,
+                   proxy_ratio=1.5)
+    kern, tree, fac, mu = O.factor_and_solve(pb.boundary, O.LAPLACE, prm,
+                                             pb.rhs)
+    K = kern.dense()
+    mud = np.linalg.solve(K, pb.rhs)
+    assert np.linalg.norm(mu - mud) < 1e-10 * np.linalg.norm(mud)
+    assert sum(n.compressed for lv in tree.levels for n in lv) > 0
+
+
+def test_factorization_error_tracks_tolerance_3d(O):
+    b = G.ellipsoid(600)
+    pb = G.make_problem(b, "stokes", 1e-5, 192, 256, 1.5, 6, 4)
+    prm = O.Params(id_tol=1e-5, max_leaf_dofs=192, proxy_points=256,
+                   proxy_ratio=1.5, proxy3d_fibonacci=True)
+    kern, tree, fac, mu = O.factor_and_solve(b, O.STOKES, prm, pb.rhs)
+    K = kern.dense()
+    res = np.linalg.norm(K @ mu - pb.rhs) / np.linalg.norm(pb.rhs)
+    assert res < 1e-3
+    assert sum(n.compressed for lv in tree.levels for n in lv) > 0
+
+
+def test_factored_inverse_times_K_is_identity(O):
+    b = G.star(32, 16)
+    prm = O.Params(id_tol=1e-12, max_leaf_dofs=64, proxy_points=128,
+                   proxy_ratio=1.5)
+    kern = O.Kernel(b, O.STOKES, prm)
+    tree = O.Tree(b.points, 2, 2, 64)
+    fac = O.SkelFactorization(prm)
+    fac.skeletonize(kern, tree)
+    K = kern.dense()
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=K.shape[0])
+    assert np.linalg.norm(fac.solve(tree, K @ x) - x) < 1e-9 * \
+        np.linalg.norm(x)
+
+
+def test_c1_interior_field_matches_exact(O):
+    pb = G.config(0)
+    prm = O.Params(id_tol=pb.id_tol, max_leaf_dofs=pb.max_leaf_dofs,
+                   proxy_points=pb.proxy_points, proxy_ratio=pb.proxy_ratio)
+    kern, tree, fac, mu = O.factor_and_solve(pb.boundary, O.LAPLACE, prm,
+                                             pb.rhs)
+    u = kern.apply_forward(pb.check_points, mu)
+    assert np.abs(u - pb.exact).max() < 1e-9 * np.abs(pb.exact).max()
+
+
+def test_descendant_update_equals_explicit_schur_sum(O):
+    """get_descendents_updates restated: the update on a parent's active set
+    equals the sum over every compressed descendant of its Schur block."""
+    pb = G.config(0)
+    prm = O.Params(id_tol=1e-10, max_leaf_dofs=64, proxy_points=64,
+                   proxy_ratio=1.5)
+    kern = O.Kernel(pb.boundary, O.LAPLACE, prm)
+    tree = O.Tree(pb.boundary.points, 2, 1, 64)
+    fac = O.SkelFactorization(prm)
+    fac.skeletonize(kern, tree)
+    root = tree.root
+    rows = root.active
+    upd = O.descendants_updates(rows, rows, root)
+    ref = np.zeros_like(upd)
+    pos = {v: i for i, v in enumerate(rows)}
+    for lv in tree.levels[1:]:
+        for nd in lv:
+            if not nd.compressed:
+                continue
+            for a, va in enumerate(nd.skel):
+                for b2, vb in enumerate(nd.skel):
+                    if va in pos and vb in pos:
+                        ref[pos[va], pos[vb]] += nd.schur[a, b2]
+    assert np.abs(upd - ref).max() < 1e-14 * max(1.0, np.abs(ref).max())
