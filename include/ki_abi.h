@@ -51,6 +51,11 @@ typedef struct ki_params {
   int qr_full;              /* 1: run every CPQR step like dgeqp3; 0: stop at
                                the step where the rank rule fires              */
   int use_predictor;        /* 1: size each level from the learned rank model  */
+  int id_method;            /* 0 auto; 1 Householder CPQR on the ID matrix;
+                               2 pivoted Cholesky of its Gram matrix (same
+                               pivot rule; auto picks 2 when tol >= 1e-7)     */
+  double fp32_tol_floor;    /* FP32 mode: ID tolerance = max(id_tol, floor);
+                               FP64 GMRES restores the requested residual     */
   int reserved[8];
 } ki_params;
 

@@ -45,6 +45,8 @@ class KiParams(C.Structure):
         ("precision", C.c_int),
         ("qr_full", C.c_int),
         ("use_predictor", C.c_int),
+        ("id_method", C.c_int),
+        ("fp32_tol_floor", C.c_double),
         ("reserved", C.c_int * 8),
     ]
 

@@ -110,7 +110,7 @@ struct Ctx {
   DevBuf top, top_piv, d_allskel, top_vec, top_desc;
   // scratch
   DevBuf idm, kbn, qws, qpiv, qrank, idxb, fdesc, qdesc, gdesc, mdesc, sdesc,
-      mapb, tmp;
+      mapb, tmp, gram, cws, cstate, cdesc;
   // vectors
   DevBuf d_b, d_x, d_xt, d_r, d_w2, d_V, d_Z, d_red, d_tmpv, d_dp, d_dout;
   ki_stats stats{};
@@ -264,6 +264,15 @@ void skeletonize_t(Ctx& c) {
   const double t_start = now();
   std::vector<int> pos(c.ndof, -1);  // DOF -> position scratch
   const bool verbose = std::getenv("KI_VERBOSE") != nullptr;
+  // FP32 cannot resolve |R_ii| / |R_00| much below 1e-6: the FP32 form
+  // compresses at max(tol, floor) and FP64 GMRES restores the residual.
+  const double tol = sizeof(T) == 4
+                         ? std::max(c.prm.id_tol, c.prm.fp32_tol_floor)
+                         : c.prm.id_tol;
+  // The Gram form resolves ratios down to ~1.5e-8 in FP64.
+  const bool use_gram =
+      c.prm.id_method == 2 || (c.prm.id_method == 0 && tol >= 1e-7);
+  std::vector<int64_t> gofs;
 
   for (int level = L - 1; level >= 1; level--) {
     // per-level log line (KI_VERBOSE), printed however the level ends
@@ -369,13 +378,55 @@ void skeletonize_t(Ctx& c) {
                        c.idxb.as<int>(), c.idm.as<T>(), s);
         st.t_fill += tm.stop();
       }
-      {
+      if (!use_gram) {
         Timer tm(s);
         launch_cpqr<T>(c.qdesc.as<QrDesc>(), nc, max_n, c.idm.as<T>(),
                        c.qpiv.as<int>(), c.qws.as<T>(), c.qrank.as<int>(),
-                       c.prm.id_tol, s);
+                       tol, s);
         launch_idsolve<T>(c.qdesc.as<QrDesc>(), nc, max_n, c.idm.as<T>(),
                           c.qrank.as<int>(), s);
+        st.t_qr += tm.stop();
+      } else {
+        // G = A^T A (FP64), pivoted Cholesky, R over G, T = R11^-1 R12
+        Timer tm(s);
+        std::vector<GramDesc> gd(nc);
+        std::vector<CholDesc> small, large;
+        std::vector<QrDesc> sd(nc);
+        int64_t g_el = 0, w_el = 0;
+        gofs.assign(nc, 0);
+        for (int q = 0; q < nc; q++) {
+          const int n = qd[q].n;
+          gd[q] = GramDesc{qd[q].m, n, qd[q].a_off, qd[q].m, g_el};
+          CholDesc ch{n, n, q, g_el, w_el, w_el + (int64_t)n * n,
+                      qd[q].piv_off};
+          (n > 640 ? large : small).push_back(ch);
+          sd[q] = QrDesc{n, n, g_el, 0, 0, n};
+          gofs[q] = g_el;
+          g_el += (int64_t)n * n;
+          w_el += (int64_t)n * n + 2 * n + 1;
+        }
+        c.gram.ensure(sizeof(double) * g_el);
+        c.cws.ensure(sizeof(double) * w_el);
+        c.cstate.ensure(sizeof(int) * 4 * (nc + 1));
+        upload(c.gdesc, gd, s);
+        launch_gram<T>(c.gdesc.as<GramDesc>(), nc, max_n, c.idm.as<T>(),
+                       c.gram.as<double>(), s);
+        if (!small.empty()) {
+          upload(c.cdesc, small, s);
+          pchol_small(c.cdesc.as<CholDesc>(), small.size(),
+                      c.gram.as<double>(), c.cws.as<double>(),
+                      c.qpiv.as<int>(), c.qrank.as<int>(), tol, s);
+          KI_HIP(hipStreamSynchronize(s));
+        }
+        if (!large.empty()) {
+          upload(c.cdesc, large, s);
+          pchol_large(large, c.cdesc.as<CholDesc>(), c.gram.as<double>(),
+                      c.cws.as<double>(), c.qpiv.as<int>(),
+                      c.cstate.as<int>(), c.qrank.as<int>(), tol, s);
+        }
+        upload(c.qdesc, sd, s);
+        launch_idsolve<double>(c.qdesc.as<QrDesc>(), nc, max_n,
+                               c.gram.as<double>(), c.qrank.as<int>(), s);
         st.t_qr += tm.stop();
       }
       KI_HIP(hipMemcpyAsync(rank.data(), c.qrank.p, sizeof(int) * nc,
@@ -581,16 +632,26 @@ void skeletonize_t(Ctx& c) {
         x = MatDesc{};
         x.m = bx.k;
         x.n = bx.r;
-        x.off = qq.a_off + (int64_t)qq.m * bx.k;
-        x.ld = qq.m;
+        if (use_gram) {  // T sits over G (FP64, ld n)
+          const int n = bx.k + bx.r;
+          x.off = gofs[comp_q[q]] + (int64_t)n * bx.k;
+          x.ld = n;
+        } else {
+          x.off = qq.a_off + (int64_t)qq.m * bx.k;
+          x.ld = qq.m;
+        }
         x.dst_off = bx.offT;
         x.ldd = bx.k;
         mm = std::max(mm, x.m);
         nn = std::max(nn, x.n);
       }
       upload(c.mdesc, md, s);
-      launch_copy<T>(c.mdesc.as<MatDesc>(), ncp, mm, nn, c.idm.as<T>(), F, 0,
-                     s);
+      if (use_gram)
+        launch_copy_cvt<double, T>(c.mdesc.as<MatDesc>(), ncp, mm, nn,
+                                   c.gram.as<double>(), F, 0, s);
+      else
+        launch_copy<T>(c.mdesc.as<MatDesc>(), ncp, mm, nn, c.idm.as<T>(), F,
+                       0, s);
     }
     // four GEMMs
     auto gemm_batch = [&](auto make, const T* A, const T* B, T* C) {
@@ -1074,6 +1135,8 @@ void ki_params_default(ki_params* p) {
   p->precision = 64;
   p->qr_full = 0;
   p->use_predictor = 0;
+  p->id_method = 0;
+  p->fp32_tol_floor = 1e-6;
 }
 
 const char* ki_last_error(void) { return ki::g_err.c_str(); }

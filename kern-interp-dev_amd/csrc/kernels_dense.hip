@@ -317,17 +317,17 @@ rhs_swap(const MatDesc* __restrict__ desc, T* __restrict__ b_all,
   }
 }
 
-// Copy (optionally transposed) m x n blocks: dst = op(src).
-template <typename T>
+// Copy (optionally transposed) m x n blocks: dst = op(src), converting type.
+template <typename TS, typename TD = TS>
 __global__ void __launch_bounds__(256)
-copy_kernel(const MatDesc* __restrict__ desc, const T* __restrict__ src,
-            T* __restrict__ dst, int transpose) {
+copy_kernel(const MatDesc* __restrict__ desc, const TS* __restrict__ src,
+            TD* __restrict__ dst, int transpose) {
   const MatDesc d = desc[blockIdx.y];
   const int64_t total = (int64_t)d.m * d.n;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int i = e % d.m, j = e / d.m;
-    const T v = src[d.off + i + (int64_t)d.ld * j];
+    const TD v = (TD)src[d.off + i + (int64_t)d.ld * j];
     if (!transpose)
       dst[d.dst_off + i + (int64_t)d.ldd * j] = v;
     else
@@ -530,10 +530,28 @@ void launch_copy(const MatDesc* d_desc, int ndesc, int max_m, int max_n,
   const int64_t tot = (int64_t)max_m * max_n;
   const int gx = (int)std::min<int64_t>((tot + 255) / 256, 4096);
   if (gx == 0) return;
-  hipLaunchKernelGGL(copy_kernel<T>, dim3(gx, ndesc), dim3(256), 0, s, d_desc,
-                     src, dst, transpose);
+  hipLaunchKernelGGL((copy_kernel<T, T>), dim3(gx, ndesc), dim3(256), 0, s,
+                     d_desc, src, dst, transpose);
   KI_HIP(hipGetLastError());
 }
+
+template <typename TS, typename TD>
+void launch_copy_cvt(const MatDesc* d_desc, int ndesc, int max_m, int max_n,
+                     const TS* src, TD* dst, int transpose, hipStream_t s) {
+  if (ndesc == 0) return;
+  const int64_t tot = (int64_t)max_m * max_n;
+  const int gx = (int)std::min<int64_t>((tot + 255) / 256, 4096);
+  if (gx == 0) return;
+  hipLaunchKernelGGL((copy_kernel<TS, TD>), dim3(gx, ndesc), dim3(256), 0, s,
+                     d_desc, src, dst, transpose);
+  KI_HIP(hipGetLastError());
+}
+template void launch_copy_cvt<double, float>(const MatDesc*, int, int, int,
+                                             const double*, float*, int,
+                                             hipStream_t);
+template void launch_copy_cvt<double, double>(const MatDesc*, int, int, int,
+                                              const double*, double*, int,
+                                              hipStream_t);
 
 template <typename T>
 void launch_zero(T* p, int64_t n, hipStream_t s) {

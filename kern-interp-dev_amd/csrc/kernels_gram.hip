@@ -1,0 +1,366 @@
+// Gram-matrix form of the interpolative decomposition.
+//
+// Column-pivoted QR of an ID matrix A picks, at step i, the column with the
+// largest norm after projecting out the columns already chosen, and |R_ii|
+// is that norm.  The same numbers are the diagonal of the Schur complements
+// of G = A^T A, so pivoted Cholesky of G (LAPACK dpstrf's rule: largest
+// remaining diagonal) selects the same skeleton and gives R = L^T, with
+// R11 = L(s,s)^T and R12 = L(r,s)^T.  In FP64 the Schur diagonals d resolve
+// |R_ii| / |R_00| down to about sqrt(eps) = 1.5e-8, so this form is used only
+// for ID tolerances >= 1e-7 (the engine falls back to the Householder CPQR
+// below that).  G costs one SYRK-shaped product per box (BLAS-3) instead of
+// the k streaming passes over A of a BLAS-2 QR.
+//
+//   gram_kernel   G = A^T A for every box of a level, FP64 accumulation,
+//                 upper-triangle tiles only (mirrored)
+//   pchol_wg      left-looking pivoted Cholesky, one workgroup per box
+//   pchol_step    one pivot step for a few large boxes, many workgroups
+//                 per box (the host loops over steps)
+#include <algorithm>
+#include <cmath>
+
+#include "ki_internal.h"
+
+namespace ki {
+
+namespace {
+
+constexpr int GT = 64, GK = 16;
+
+template <typename TI>
+__global__ void __launch_bounds__(256)
+gram_kernel(const GramDesc* __restrict__ desc, const TI* __restrict__ a_all,
+            double* __restrict__ g_all) {
+  const GramDesc d = desc[blockIdx.y];
+  const int tn = (d.n + GT - 1) / GT;
+  // upper-triangular tile enumeration: tile (ti, tj) with ti <= tj
+  const int t = blockIdx.x;
+  if (t >= tn * (tn + 1) / 2) return;
+  int tj = (int)((sqrt(8.0 * t + 1.0) - 1.0) / 2.0);
+  while ((tj + 1) * (tj + 2) / 2 <= t) tj++;
+  while (tj * (tj + 1) / 2 > t) tj--;
+  const int ti = t - tj * (tj + 1) / 2;
+  const int i0 = ti * GT, j0 = tj * GT;
+  const TI* A = a_all + d.a_off;
+  __shared__ double As[GK][GT + 1];
+  __shared__ double Bs[GK][GT + 1];
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[a][b] = 0;
+  for (int k0 = 0; k0 < d.m; k0 += GK) {
+    for (int e = threadIdx.x; e < GT * GK; e += 256) {
+      const int kk = e % GK, ii = e / GK;
+      const int gk = k0 + kk;
+      const int gi = i0 + ii, gj = j0 + ii;
+      As[kk][ii] = (gk < d.m && gi < d.n)
+                       ? (double)A[gk + (int64_t)d.lda * gi]
+                       : 0.0;
+      Bs[kk][ii] = (gk < d.m && gj < d.n)
+                       ? (double)A[gk + (int64_t)d.lda * gj]
+                       : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GK; kk++) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int a = 0; a < 4; a++) av[a] = As[kk][tx + 16 * a];
+#pragma unroll
+      for (int b = 0; b < 4; b++) bv[b] = Bs[kk][ty + 16 * b];
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] += av[a] * bv[b];
+    }
+    __syncthreads();
+  }
+  double* G = g_all + d.g_off;
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    const int gj = j0 + ty + 16 * b;
+    if (gj >= d.n) continue;
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      const int gi = i0 + tx + 16 * a;
+      if (gi >= d.n) continue;
+      G[gi + (int64_t)d.n * gj] = acc[a][b];
+      G[gj + (int64_t)d.n * gi] = acc[a][b];
+    }
+  }
+}
+
+// Argmax of the Schur diagonal over unselected columns (selected columns
+// carry the sentinel -1; the first index wins ties, as idamax).
+__device__ __forceinline__ void argmax_dev(const double* dg, int n,
+                                           double* redv, int* redi,
+                                           double& best_out, int& idx_out) {
+  double best = -0.5;
+  int bi = n;
+  for (int j = threadIdx.x; j < n; j += 256) {
+    const double v = dg[j];
+    if (v > best) {
+      best = v;
+      bi = j;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ov = __shfl_xor(best, o, 64);
+    const int oi = __shfl_xor(bi, o, 64);
+    if (ov > best || (ov == best && oi < bi)) {
+      best = ov;
+      bi = oi;
+    }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+    redv[wid] = best;
+    redi[wid] = bi;
+  }
+  __syncthreads();
+  double b = redv[0];
+  int i = redi[0];
+  for (int w = 1; w < 4; w++)
+    if (redv[w] > b || (redv[w] == b && redi[w] < i)) {
+      b = redv[w];
+      i = redi[w];
+    }
+  best_out = b;
+  idx_out = i;
+}
+
+// R(a, c) = L(piv[c], a) for a < rank (upper trapezoid over the first rank
+// rows of G, skeleton-first column order); redundant columns follow the
+// skeleton in ascending original order.
+__device__ void write_r(double* G, const double* L, const double* dg, int* piv,
+                        int n, int rank) {
+  if (threadIdx.x == 0) {
+    int q = rank;
+    for (int j = 0; j < n; j++)
+      if (dg[j] >= 0.0) piv[q++] = j;
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < n; c += 256) {
+    const int pc = piv[c];
+    for (int a = 0; a < rank; a++)
+      G[a + (int64_t)n * c] =
+          (a <= c || c >= rank) ? L[pc + (int64_t)n * a] : 0.0;
+  }
+}
+
+// Left-looking pivoted Cholesky of G (n x n, ld n) with the reference's rank
+// rule, one workgroup per box.  L is column-major n x kmax in the workspace
+// (original column indexing).
+__global__ void __launch_bounds__(256)
+pchol_wg(const CholDesc* __restrict__ desc, double* __restrict__ g_all,
+         double* __restrict__ ws_all, int* __restrict__ piv_all,
+         int* __restrict__ rank_out, double tol) {
+  const CholDesc d = desc[blockIdx.x];
+  const int n = d.n;
+  double* G = g_all + d.g_off;
+  double* L = ws_all + d.l_off;
+  double* dg = ws_all + d.d_off;
+  int* piv = piv_all + d.piv_off;
+  __shared__ double redv[4];
+  __shared__ int redi[4];
+  for (int j = threadIdx.x; j < n; j += 256) dg[j] = G[j + (int64_t)n * j];
+  __syncthreads();
+  const int kmax = min(d.kmax, n);
+  int rank = 0;
+  double thresh = 0;
+  for (int i = 0; i < kmax; i++) {
+    double best;
+    int p;
+    argmax_dev(dg, n, redv, redi, best, p);
+    if (p >= n) break;
+    const double rii = sqrt(fmax(best, 0.0));
+    if (i == 0) {
+      thresh = fabs(tol * rii);
+    } else if (rii < thresh) {
+      rank = i;
+      break;
+    }
+    if (rii == 0.0) break;
+    const double inv = 1.0 / rii;
+    for (int j = threadIdx.x; j < n; j += 256) {
+      if (dg[j] < 0.0) continue;  // selected earlier
+      if (j == p) {
+        L[j + (int64_t)n * i] = rii;
+        continue;
+      }
+      double sm = G[j + (int64_t)n * p];
+      for (int l = 0; l < i; l++)
+        sm -= L[j + (int64_t)n * l] * L[p + (int64_t)n * l];
+      const double v = sm * inv;
+      L[j + (int64_t)n * i] = v;
+      dg[j] = fmax(dg[j] - v * v, 0.0);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      piv[i] = p;
+      dg[p] = -1.0;
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) rank_out[d.slot] = rank;
+  if (rank == 0) return;
+  write_r(G, L, dg, piv, n, rank);
+}
+
+// One pivot step i for each large box (grid.y = box, grid.x = row chunks).
+// The Schur diagonal is double-buffered (din / dout) so that every
+// workgroup of a step reads the same values and picks the same pivot.
+__global__ void __launch_bounds__(256)
+pchol_step(const CholDesc* __restrict__ desc, const double* __restrict__ g_all,
+           double* __restrict__ ws_all, int* __restrict__ piv_all,
+           int* __restrict__ state, int i, int parity, double tol) {
+  const CholDesc d = desc[blockIdx.y];
+  const int n = d.n;
+  int* st = state + 4 * d.slot;  // [rank, done, -, -]
+  if (st[1]) return;
+  if (i >= min(d.kmax, n)) return;
+  const double* G = g_all + d.g_off;
+  double* L = ws_all + d.l_off;
+  const double* din = ws_all + d.d_off + (parity ? n : 0);
+  double* dout = ws_all + d.d_off + (parity ? 0 : n);
+  int* piv = piv_all + d.piv_off;
+  __shared__ double redv[4];
+  __shared__ int redi[4];
+  double best;
+  int p;
+  argmax_dev(din, n, redv, redi, best, p);
+  if (p >= n) return;
+  const double rii = sqrt(fmax(best, 0.0));
+  const double thresh = i == 0 ? fabs(tol * rii) : ws_all[d.d_off + 2 * n];
+  const bool crit = i > 0 && rii < thresh;
+  if (crit || rii == 0.0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st[0] = crit ? i : 0;
+      st[1] = 1;
+    }
+    return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    piv[i] = p;
+    if (i == 0) ws_all[d.d_off + 2 * n] = thresh;
+  }
+  const double inv = 1.0 / rii;
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  const double dj = din[j];
+  if (dj < 0.0) {
+    dout[j] = dj;
+    return;
+  }
+  if (j == p) {
+    L[j + (int64_t)n * i] = rii;
+    dout[j] = -1.0;
+    return;
+  }
+  double sm = G[j + (int64_t)n * p];
+  for (int l = 0; l < i; l++)
+    sm -= L[j + (int64_t)n * l] * L[p + (int64_t)n * l];
+  const double v = sm * inv;
+  L[j + (int64_t)n * i] = v;
+  dout[j] = fmax(dj - v * v, 0.0);
+}
+
+// Initialise the large-box state: d = diag(G) in the even buffer.
+__global__ void pchol_init(const CholDesc* __restrict__ desc,
+                           const double* __restrict__ g_all,
+                           double* __restrict__ ws_all,
+                           int* __restrict__ state) {
+  const CholDesc d = desc[blockIdx.y];
+  const int n = d.n;
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j == 0) {
+    state[4 * d.slot] = 0;
+    state[4 * d.slot + 1] = 0;
+  }
+  if (j >= n) return;
+  ws_all[d.d_off + j] = g_all[d.g_off + j + (int64_t)n * j];
+}
+
+// Finish the large boxes: rank, redundant order, R = L^T over G.
+__global__ void __launch_bounds__(256)
+pchol_finish(const CholDesc* __restrict__ desc, double* __restrict__ g_all,
+             double* __restrict__ ws_all, int* __restrict__ piv_all,
+             const int* __restrict__ state, int* __restrict__ rank_out,
+             int steps_run) {
+  const CholDesc d = desc[blockIdx.x];
+  const int n = d.n;
+  const int rank = state[4 * d.slot];
+  if (threadIdx.x == 0) rank_out[d.slot] = rank;
+  if (rank == 0) return;
+  // the live diagonal buffer is the one written by the last executed step
+  const int last = rank;  // steps 0..rank-1 ran; step `rank` only stopped
+  const double* dg = ws_all + d.d_off + ((last & 1) ? n : 0);
+  (void)steps_run;
+  write_r(g_all + d.g_off, ws_all + d.l_off, dg, piv_all + d.piv_off, n, rank);
+}
+
+}  // namespace
+
+template <typename TI>
+void launch_gram(const GramDesc* d_desc, int ndesc, int max_n, const TI* a,
+                 double* g, hipStream_t s) {
+  if (ndesc == 0 || max_n == 0) return;
+  const int tn = (max_n + GT - 1) / GT;
+  const int tiles = tn * (tn + 1) / 2;
+  for (int base = 0; base < ndesc; base += 65535) {
+    const int cnt = std::min(65535, ndesc - base);
+    hipLaunchKernelGGL(gram_kernel<TI>, dim3(tiles, cnt), dim3(256), 0, s,
+                       d_desc + base, a, g);
+  }
+  KI_HIP(hipGetLastError());
+}
+
+void pchol_small(const CholDesc* d_desc, int ndesc, double* g, double* ws,
+                 int* piv, int* rank, double tol, hipStream_t s) {
+  if (ndesc == 0) return;
+  hipLaunchKernelGGL(pchol_wg, dim3(ndesc), dim3(256), 0, s, d_desc, g, ws,
+                     piv, rank, tol);
+  KI_HIP(hipGetLastError());
+}
+
+void pchol_large(const std::vector<CholDesc>& h, const CholDesc* d_desc,
+                 double* g, double* ws, int* piv, int* state, int* rank,
+                 double tol, hipStream_t s) {
+  const int nb = h.size();
+  if (nb == 0) return;
+  int maxn = 0, maxsteps = 0;
+  for (auto& x : h) {
+    maxn = std::max(maxn, x.n);
+    maxsteps = std::max(maxsteps, std::min(x.kmax, x.n));
+  }
+  const dim3 grid((maxn + 255) / 256, nb);
+  hipLaunchKernelGGL(pchol_init, grid, dim3(256), 0, s, d_desc, g, ws, state);
+  std::vector<int> hst(4 * (size_t)nb);
+  int i = 0;
+  for (; i < maxsteps; i++) {
+    hipLaunchKernelGGL(pchol_step, grid, dim3(256), 0, s, d_desc, g, ws, piv,
+                       state, i, i & 1, tol);
+    if ((i & 63) == 63) {  // stop early once every box has met its rank
+      KI_HIP(hipMemcpyAsync(hst.data(), state, sizeof(int) * 4 * nb,
+                            hipMemcpyDeviceToHost, s));
+      KI_HIP(hipStreamSynchronize(s));
+      bool all = true;
+      for (int b = 0; b < nb; b++) all = all && hst[4 * b + 1];
+      if (all) break;
+    }
+  }
+  hipLaunchKernelGGL(pchol_finish, dim3(nb), dim3(256), 0, s, d_desc, g, ws,
+                     piv, state, rank, i);
+  KI_HIP(hipGetLastError());
+}
+
+template void launch_gram<float>(const GramDesc*, int, int, const float*,
+                                 double*, hipStream_t);
+template void launch_gram<double>(const GramDesc*, int, int, const double*,
+                                  double*, hipStream_t);
+
+}  // namespace ki

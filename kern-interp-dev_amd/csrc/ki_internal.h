@@ -170,6 +170,31 @@ template <typename T>
 void cpqr_large(const std::vector<QrDesc>& desc, T* a, int* piv, T* ws,
                 int* d_rank, int* h_rank, double tol, hipStream_t s);
 
+// Gram form of the ID: G = A^T A (FP64) and pivoted Cholesky of G.
+struct GramDesc {
+  int m, n;
+  int64_t a_off;
+  int lda;
+  int64_t g_off;     // n x n, ld n (FP64)
+};
+
+struct CholDesc {
+  int n, kmax, slot;  // slot: index of the box in the level's rank array
+  int64_t g_off;      // G (n x n, ld n); R is written over its first k rows
+  int64_t l_off;      // L workspace (n x kmax)
+  int64_t d_off;      // Schur diagonal workspace (2n + 1)
+  int64_t piv_off;    // n ints
+};
+
+template <typename TI>
+void launch_gram(const GramDesc* d_desc, int ndesc, int max_n, const TI* a,
+                 double* g, hipStream_t s);
+void pchol_small(const CholDesc* d_desc, int ndesc, double* g, double* ws,
+                 int* piv, int* rank, double tol, hipStream_t s);
+void pchol_large(const std::vector<CholDesc>& h, const CholDesc* d_desc,
+                 double* g, double* ws, int* piv, int* state, int* rank,
+                 double tol, hipStream_t s);
+
 // T = R11^{-1} R12 in place (rows 0..k-1, cols k..n-1 of the QR output).
 template <typename T>
 void launch_idsolve(const QrDesc* d_desc, int ndesc, int max_n, T* a,
@@ -214,6 +239,9 @@ void batched_getrs(const std::vector<MatDesc>& h, MatDesc* d_desc,
 template <typename T>
 void launch_copy(const MatDesc* d_desc, int ndesc, int max_m, int max_n,
                  const T* src, T* dst, int transpose, hipStream_t s);
+template <typename TS, typename TD>
+void launch_copy_cvt(const MatDesc* d_desc, int ndesc, int max_m, int max_n,
+                     const TS* src, TD* dst, int transpose, hipStream_t s);
 
 template <typename T>
 void launch_zero(T* p, int64_t n, hipStream_t s);

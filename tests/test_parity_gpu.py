@@ -158,6 +158,23 @@ def _oracle_factor(name):
     return _cache[key]
 
 
+def _gpu_id_matrix(ctx, okern, tree, l, i):
+    """The oracle's ID matrix for box (l, i) built from the GPU's lists."""
+    from types import SimpleNamespace as NS
+    levels = []
+    for lv in range(len(tree.levels)):
+        row = []
+        for j, nd in enumerate(tree.levels[lv]):
+            row.append(NS(active=list(ctx.box_list(lv, j, "active")),
+                          original=nd.original, is_leaf=nd.is_leaf))
+        levels.append(row)
+    nd = tree.levels[l][i]
+    me = levels[l][i]
+    me.level, me.center, me.side = l, nd.center, nd.side
+    me.near = list(ctx.box_list(l, i, "near"))
+    return okern.id_mat(NS(levels=levels), me), me.active
+
+
 @pytest.mark.parametrize("name", CASES)
 def test_skeletons_match_oracle_fp64(name):
     """Plain FP64 GPU path: same compressed boxes, ranks and skeleton sets as
@@ -180,19 +197,20 @@ def test_skeletons_match_oracle_fp64(name):
             if set(sk) == set(nd.skel):
                 same += 1
                 continue
-            # A different skeleton is accepted only as a pivot tie: on the
-            # same ID matrix (same active set) it must interpolate the
-            # redundant columns as well as the oracle's skeleton does.
-            act = list(ctx.box_list(l, i, "active"))
-            assert act == nd.active, (l, i, "active sets differ")
-            A = okern.id_mat(tree, nd)
+            # A different skeleton is accepted only as a pivot tie: the ID
+            # matrix is rebuilt from the GPU's own lists (a tie below changes
+            # this box's active set), the oracle's dgeqp3 ID is run on it,
+            # and the GPU skeleton must interpolate the redundant columns as
+            # well as that ID does.
+            A, act = _gpu_id_matrix(ctx, okern, tree, l, i)
+            k_o, p_o, _ = O.interpolative_decomposition(A, kw["id_tol"])
+            assert k_o == len(sk), (l, i, k_o, len(sk))
             pos = {v: q for q, v in enumerate(act)}
             s_idx = [pos[v] for v in sk]
             r_idx = [q for q in range(len(act)) if q not in set(s_idx)]
             Tls = np.linalg.lstsq(A[:, s_idx], A[:, r_idx], rcond=None)[0]
             err = np.linalg.norm(A[:, r_idx] - A[:, s_idx] @ Tls, 2)
-            so = [pos[v] for v in nd.skel]
-            ro = [q for q in range(len(act)) if q not in set(so)]
+            so, ro = list(p_o[:k_o]), list(p_o[k_o:])
             Tor = np.linalg.lstsq(A[:, so], A[:, ro], rcond=None)[0]
             err_o = np.linalg.norm(A[:, ro] - A[:, so] @ Tor, 2)
             assert err <= max(10 * err_o, 10 * kw["id_tol"] *
