@@ -383,15 +383,12 @@ void skeletonize_t(Ctx& c) {
         launch_cpqr<T>(c.qdesc.as<QrDesc>(), nc, max_n, c.idm.as<T>(),
                        c.qpiv.as<int>(), c.qws.as<T>(), c.qrank.as<int>(),
                        tol, s);
-        launch_idsolve<T>(c.qdesc.as<QrDesc>(), nc, max_n, c.idm.as<T>(),
-                          c.qrank.as<int>(), s);
         st.t_qr += tm.stop();
       } else {
         // G = A^T A (FP64), pivoted Cholesky, R over G, T = R11^-1 R12
         Timer tm(s);
         std::vector<GramDesc> gd(nc);
         std::vector<CholDesc> small, large;
-        std::vector<QrDesc> sd(nc);
         int64_t g_el = 0, w_el = 0;
         gofs.assign(nc, 0);
         for (int q = 0; q < nc; q++) {
@@ -400,7 +397,6 @@ void skeletonize_t(Ctx& c) {
           CholDesc ch{n, n, q, g_el, w_el, w_el + (int64_t)n * n,
                       qd[q].piv_off};
           (n > 640 ? large : small).push_back(ch);
-          sd[q] = QrDesc{n, n, g_el, 0, 0, n};
           gofs[q] = g_el;
           g_el += (int64_t)n * n;
           w_el += (int64_t)n * n + 2 * n + 1;
@@ -424,9 +420,6 @@ void skeletonize_t(Ctx& c) {
                       c.cws.as<double>(), c.qpiv.as<int>(),
                       c.cstate.as<int>(), c.qrank.as<int>(), tol, s);
         }
-        upload(c.qdesc, sd, s);
-        launch_idsolve<double>(c.qdesc.as<QrDesc>(), nc, max_n,
-                               c.gram.as<double>(), c.qrank.as<int>(), s);
         st.t_qr += tm.stop();
       }
       KI_HIP(hipMemcpyAsync(rank.data(), c.qrank.p, sizeof(int) * nc,
@@ -464,6 +457,38 @@ void skeletonize_t(Ctx& c) {
       lv_sumk += k;
       c.boxlog.insert(c.boxlog.end(), {level, n, qd[q].m, k});
       st.boxes_compressed++;
+    }
+    // ---- T = R11^{-1} R12 over R12 (blocked triangular solve) ----------
+    if (!comp.empty()) {
+      Timer tm(s);
+      std::vector<MatDesc> md(comp.size());
+      for (size_t q = 0; q < comp.size(); q++) {
+        const Box& bx = tr.boxes[comp[q]];
+        const int cq = comp_q[q];
+        const int n = bx.k + bx.r;
+        MatDesc x{};
+        x.m = x.n = bx.k;
+        x.nrhs = bx.r;
+        if (use_gram) {
+          x.off = gofs[cq];
+          x.ld = n;
+        } else {
+          x.off = qd[cq].a_off;
+          x.ld = qd[cq].m;
+        }
+        x.b_off = x.off + (int64_t)x.ld * bx.k;
+        x.ldb = x.ld;
+        md[q] = x;
+      }
+      upload(c.mdesc, md, s);
+      if (use_gram)
+        batched_trsm_upper<double>(md, c.mdesc.as<MatDesc>(),
+                                   c.gram.as<double>(), c.gram.as<double>(),
+                                   s);
+      else
+        batched_trsm_upper<T>(md, c.mdesc.as<MatDesc>(), c.idm.as<T>(),
+                              c.idm.as<T>(), s);
+      st.t_qr += tm.stop();
     }
     // ---- factor storage: compressed boxes and pass-through updates -----
     LevelDev& LD = *c.lev[level];

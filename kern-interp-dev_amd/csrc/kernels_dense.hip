@@ -10,6 +10,7 @@
 // sweep (:585-592, 676-722).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "ki_internal.h"
 
@@ -199,6 +200,87 @@ lu_panel(const MatDesc* __restrict__ desc, T* __restrict__ a_all,
   }
 }
 
+// One column step c of the panel factorization of a single tall matrix,
+// spread over many workgroups (one row per thread).  The panel is double-
+// buffered between A (parity 0 reads A) and the scratch W (ld = m - j0), so
+// every workgroup reads the same values: each finds the same pivot row p,
+// then writes its rows of the next buffer with rows c and p interchanged and
+// the rank-1 update applied.  piv follows LAPACK (sequential interchanges).
+template <typename T>
+__global__ void __launch_bounds__(256)
+lu_pstep(T* __restrict__ A, int ld, int m, int j0, int jb,
+         T* __restrict__ W, int c, int* __restrict__ piv, int parity) {
+  const int mr = m - j0;
+  const int ldw = mr;
+  auto at = [&](int r, int q) -> T {
+    return parity == 0 ? A[(j0 + r) + (int64_t)ld * (j0 + q)]
+                       : W[r + (int64_t)ldw * q];
+  };
+  // pivot: first row of maximal |a| in column c, rows c..mr-1
+  T best = -1;
+  int bi = mr;
+  for (int r = c + threadIdx.x; r < mr; r += 256) {
+    const T v = fabs(at(r, c));
+    if (v > best) {
+      best = v;
+      bi = r;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const T ov = __shfl_xor(best, o, 64);
+    const int oi = __shfl_xor(bi, o, 64);
+    if (ov > best || (ov == best && oi < bi)) {
+      best = ov;
+      bi = oi;
+    }
+  }
+  __shared__ T rv[4];
+  __shared__ int ri[4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    rv[wid] = best;
+    ri[wid] = bi;
+  }
+  __syncthreads();
+  T b = rv[0];
+  int p = ri[0];
+  for (int w = 1; w < 4; w++)
+    if (rv[w] > b || (rv[w] == b && ri[w] < p)) {
+      b = rv[w];
+      p = ri[w];
+    }
+  if (p < c || p >= mr) p = c;  // non-finite column: no interchange
+  if (blockIdx.x == 0 && threadIdx.x == 0) piv[j0 + c] = j0 + p;
+  const T pv = at(p, c);
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= mr) return;
+  const int sr = r == c ? p : (r == p ? c : r);
+  const T l = (r > c && pv != (T)0) ? at(sr, c) / pv : (T)0;
+  for (int q = 0; q < jb; q++) {
+    T x = at(sr, q);
+    if (r > c) {
+      if (q == c)
+        x = l;
+      else if (q > c)
+        x -= l * at(p, q);
+    }
+    if (parity == 0)
+      W[r + (int64_t)ldw * q] = x;
+    else
+      A[(j0 + r) + (int64_t)ld * (j0 + q)] = x;
+  }
+}
+
+template <typename T>
+__global__ void panel_copy_back(T* __restrict__ A, int ld, int m, int j0,
+                                int jb, const T* __restrict__ W) {
+  const int mr = m - j0;
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= mr) return;
+  for (int q = 0; q < jb; q++)
+    A[(j0 + r) + (int64_t)ld * (j0 + q)] = W[r + (int64_t)mr * q];
+}
+
 // Apply the panel's row interchanges to the columns outside the panel, then
 // A12 <- L11^{-1} A12 for the columns to the right (one thread per column).
 template <typename T>
@@ -237,8 +319,9 @@ lu_swap_trsm(const MatDesc* __restrict__ desc, T* __restrict__ a_all,
 // Triangular solves with the LU factors, blocked by LB rows:
 //   mode 0: L (unit lower) forward      mode 1: U (upper) backward
 //   mode 2: U^T (lower) forward         mode 3: L^T (unit upper) backward
-// The diagonal-block kernel solves one LB block for every right-hand side
-// column (one thread per column); GEMM updates the remaining rows.
+// The diagonal-block kernel stages the LB x LB triangle in LDS and solves it
+// for every right-hand-side column (one thread per column, the column's LB
+// entries in registers); GEMM updates the remaining rows.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -247,42 +330,54 @@ trsm_diag(const MatDesc* __restrict__ desc, const T* __restrict__ a_all,
   const MatDesc d = desc[blockIdx.y];
   const int n = d.n;
   const int nblk = (n + LB - 1) / LB;
-  if (blk >= nblk) return;
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= d.nrhs) return;
+  if (blk >= nblk || d.nrhs <= 0) return;
   // forward modes walk blocks top-down, backward modes bottom-up
   const int b = (mode == 0 || mode == 2) ? blk : nblk - 1 - blk;
   const int r0 = b * LB, rb = min(LB, n - r0);
   const T* A = a_all + d.off;
   const int ld = d.ld;
-  T* x = b_all + d.b_off + (int64_t)d.ldb * col;
-  if (mode == 0) {
-    for (int q = 0; q < rb; q++) {
-      const T v = x[r0 + q];
-      for (int t = q + 1; t < rb; t++)
-        x[r0 + t] -= A[(r0 + t) + (int64_t)ld * (r0 + q)] * v;
-    }
-  } else if (mode == 1) {
-    for (int q = rb - 1; q >= 0; q--) {
-      const T v = x[r0 + q] / A[(r0 + q) + (int64_t)ld * (r0 + q)];
-      x[r0 + q] = v;
-      for (int t = 0; t < q; t++)
-        x[r0 + t] -= A[(r0 + t) + (int64_t)ld * (r0 + q)] * v;
-    }
-  } else if (mode == 2) {  // U^T lower: (U^T)(t,q) = U(q,t)
-    for (int q = 0; q < rb; q++) {
-      const T v = x[r0 + q] / A[(r0 + q) + (int64_t)ld * (r0 + q)];
-      x[r0 + q] = v;
-      for (int t = q + 1; t < rb; t++)
-        x[r0 + t] -= A[(r0 + q) + (int64_t)ld * (r0 + t)] * v;
-    }
-  } else {  // L^T unit upper: (L^T)(t,q) = L(q,t)
-    for (int q = rb - 1; q >= 0; q--) {
-      const T v = x[r0 + q];
-      for (int t = 0; t < q; t++)
-        x[r0 + t] -= A[(r0 + q) + (int64_t)ld * (r0 + t)] * v;
-    }
+  // D[t][q] = A(r0 + t, r0 + q); transposed modes read D[q][t]
+  __shared__ T D[LB][LB + 1];
+  for (int e = threadIdx.x; e < LB * LB; e += blockDim.x) {
+    const int t = e % LB, q = e / LB;  // identity outside the rb x rb block
+    D[t][q] = (t < rb && q < rb) ? A[(r0 + t) + (int64_t)ld * (r0 + q)]
+                                 : (T)(t == q ? 1 : 0);
   }
+  __syncthreads();
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= d.nrhs) return;
+  T* xp = b_all + d.b_off + (int64_t)d.ldb * col + r0;
+  T x[LB];
+#pragma unroll
+  for (int q = 0; q < LB; q++) x[q] = q < rb ? xp[q] : (T)0;
+  if (mode == 0) {  // unit lower
+#pragma unroll
+    for (int q = 0; q < LB; q++)
+#pragma unroll
+      for (int t = q + 1; t < LB; t++) x[t] -= D[t][q] * x[q];
+  } else if (mode == 1) {  // upper, non-unit
+#pragma unroll
+    for (int q = LB - 1; q >= 0; q--) {
+      if (q < rb) x[q] /= D[q][q];
+#pragma unroll
+      for (int t = 0; t < q; t++) x[t] -= D[t][q] * x[q];
+    }
+  } else if (mode == 2) {  // U^T: lower, non-unit, (U^T)(t,q) = D[q][t]
+#pragma unroll
+    for (int q = 0; q < LB; q++) {
+      if (q < rb) x[q] /= D[q][q];
+#pragma unroll
+      for (int t = q + 1; t < LB; t++) x[t] -= D[q][t] * x[q];
+    }
+  } else {  // L^T: unit upper, (L^T)(t,q) = D[q][t]
+#pragma unroll
+    for (int q = LB - 1; q >= 0; q--)
+#pragma unroll
+      for (int t = 0; t < q; t++) x[t] -= D[q][t] * x[q];
+  }
+#pragma unroll
+  for (int q = 0; q < LB; q++)
+    if (q < rb) xp[q] = x[q];
 }
 
 // Row interchanges on a right-hand-side block (laswp), forward or reverse.
@@ -419,10 +514,34 @@ void batched_lu(const std::vector<MatDesc>& h, MatDesc* d_desc, T* a, int* piv,
   KI_HIP(hipMallocAsync((void**)&d_g, sizeof(GemmDesc) * gd.size(), s));
   KI_HIP(hipMemcpyAsync(d_g, gd.data(), sizeof(GemmDesc) * gd.size(),
                         hipMemcpyHostToDevice, s));
+  // A single tall matrix (the top-level block) factors its panels with the
+  // multi-workgroup step kernel; batches use one workgroup per panel.
+  static const int wide_rows = [] {
+    const char* e = std::getenv("KI_WIDE_LU_ROWS");  // test override
+    return e ? std::atoi(e) : 4096;
+  }();
+  const bool wide = nb == 1 && h[0].m > wide_rows;
+  T* W = nullptr;
+  if (wide)
+    KI_HIP(hipMallocAsync((void**)&W, sizeof(T) * (size_t)h[0].m * LB, s));
   for (int p = 0; p < npan; p++) {
     const int j0 = p * LB;
-    hipLaunchKernelGGL(lu_panel<T>, dim3(nb), dim3(256), 0, s, d_desc, a, piv,
-                       j0);
+    if (wide) {
+      const MatDesc& x = h[0];
+      const int jb = std::min(LB, std::min(x.m, x.n) - j0);
+      const int mr = x.m - j0;
+      const dim3 g((mr + 255) / 256);
+      T* A0 = a + x.off;
+      for (int c = 0; c < jb; c++)
+        hipLaunchKernelGGL(lu_pstep<T>, g, dim3(256), 0, s, A0, x.ld, x.m, j0,
+                           jb, W, c, piv + x.piv_off, c & 1);
+      if (jb & 1)
+        hipLaunchKernelGGL(panel_copy_back<T>, g, dim3(256), 0, s, A0, x.ld,
+                           x.m, j0, jb, (const T*)W);
+    } else {
+      hipLaunchKernelGGL(lu_panel<T>, dim3(nb), dim3(256), 0, s, d_desc, a,
+                         piv, j0);
+    }
     hipLaunchKernelGGL(lu_swap_trsm<T>, dim3((maxn + 255) / 256, nb),
                        dim3(256), 0, s, d_desc, a, piv, j0);
     if (mm[p] > 0 && nn[p] > 0)
@@ -432,11 +551,16 @@ void batched_lu(const std::vector<MatDesc>& h, MatDesc* d_desc, T* a, int* piv,
   // the host descriptor vector dies here: make sure the upload has landed
   KI_HIP(hipStreamSynchronize(s));
   KI_HIP(hipFreeAsync(d_g, s));
+  if (W) KI_HIP(hipFreeAsync(W, s));
 }
 
+// Blocked triangular sweeps (modes as in trsm_diag) on every box, with the
+// row interchanges of the LU applied before (swap = 1) or after (swap = 2)
+// the sweeps, or not at all (swap = 0).
 template <typename T>
-void batched_getrs(const std::vector<MatDesc>& h, MatDesc* d_desc, const T* a,
-                   const int* piv, T* b, bool trans, hipStream_t s) {
+static void trsm_sweeps(const std::vector<MatDesc>& h, MatDesc* d_desc,
+                        const T* a, const int* piv, T* b, const int* modes,
+                        int nmodes, int swap, hipStream_t s) {
   if (h.empty()) return;
   const int nb = h.size();
   int maxn = 0, maxr = 0;
@@ -446,11 +570,10 @@ void batched_getrs(const std::vector<MatDesc>& h, MatDesc* d_desc, const T* a,
   }
   if (maxr == 0) return;
   const int nblk = (maxn + LB - 1) / LB;
-  const int modes[2] = {trans ? 2 : 0, trans ? 3 : 1};
-  // GEMM descriptors for both triangular sweeps, every block, uploaded once.
-  std::vector<GemmDesc> gd((size_t)2 * nblk * nb);
-  std::vector<int> mm(2 * nblk, 0), nn(2 * nblk, 0);
-  for (int mi = 0; mi < 2; mi++) {
+  // GEMM descriptors for every sweep and block, uploaded once.
+  std::vector<GemmDesc> gd((size_t)nmodes * nblk * nb);
+  std::vector<int> mm(nmodes * nblk, 0), nn(nmodes * nblk, 0);
+  for (int mi = 0; mi < nmodes; mi++) {
     const int mode = modes[mi];
     const bool fwd = (mode == 0 || mode == 2);
     for (int blk = 0; blk < nblk; blk++) {
@@ -504,9 +627,9 @@ void batched_getrs(const std::vector<MatDesc>& h, MatDesc* d_desc, const T* a,
   KI_HIP(hipMemcpyAsync(d_g, gd.data(), sizeof(GemmDesc) * gd.size(),
                         hipMemcpyHostToDevice, s));
   const dim3 cgrid((maxr + 255) / 256, nb);
-  if (!trans)
+  if (swap == 1)
     hipLaunchKernelGGL(rhs_swap<T>, cgrid, dim3(256), 0, s, d_desc, b, piv, 0);
-  for (int mi = 0; mi < 2; mi++) {
+  for (int mi = 0; mi < nmodes; mi++) {
     for (int blk = 0; blk < nblk; blk++) {
       const int slot = mi * nblk + blk;
       hipLaunchKernelGGL(trsm_diag<T>, cgrid, dim3(256), 0, s, d_desc, a, b,
@@ -516,11 +639,25 @@ void batched_getrs(const std::vector<MatDesc>& h, MatDesc* d_desc, const T* a,
                        b, s);
     }
   }
-  if (trans)
+  if (swap == 2)
     hipLaunchKernelGGL(rhs_swap<T>, cgrid, dim3(256), 0, s, d_desc, b, piv, 1);
   KI_HIP(hipGetLastError());
   KI_HIP(hipStreamSynchronize(s));
   KI_HIP(hipFreeAsync(d_g, s));
+}
+
+template <typename T>
+void batched_getrs(const std::vector<MatDesc>& h, MatDesc* d_desc, const T* a,
+                   const int* piv, T* b, bool trans, hipStream_t s) {
+  const int modes[2] = {trans ? 2 : 0, trans ? 3 : 1};
+  trsm_sweeps<T>(h, d_desc, a, piv, b, modes, 2, trans ? 2 : 1, s);
+}
+
+template <typename T>
+void batched_trsm_upper(const std::vector<MatDesc>& h, MatDesc* d_desc,
+                        const T* a, T* b, hipStream_t s) {
+  const int modes[1] = {1};
+  trsm_sweeps<T>(h, d_desc, a, nullptr, b, modes, 1, 0, s);
 }
 
 template <typename T>
@@ -579,6 +716,8 @@ void launch_scatter(const ScatterDesc* d_desc, int ndesc, int max_n,
                                const T*, T*, hipStream_t);                    \
   template void batched_lu<T>(const std::vector<MatDesc>&, MatDesc*, T*,      \
                               int*, hipStream_t);                             \
+  template void batched_trsm_upper<T>(const std::vector<MatDesc>&, MatDesc*,  \
+                                      const T*, T*, hipStream_t);             \
   template void batched_getrs<T>(const std::vector<MatDesc>&, MatDesc*,       \
                                  const T*, const int*, T*, bool, hipStream_t); \
   template void launch_copy<T>(const MatDesc*, int, int, int, const T*, T*,   \

@@ -248,9 +248,22 @@ pchol_step(const CholDesc* __restrict__ desc, const double* __restrict__ g_all,
     piv[i] = p;
     if (i == 0) ws_all[d.d_off + 2 * n] = thresh;
   }
-  const double inv = 1.0 / rii;
-  const int j = blockIdx.x * 256 + threadIdx.x;
-  if (j >= n) return;
+  // This workgroup owns columns j0..j0+63 (one per lane); the four waves
+  // split the sum over l = 0..i-1 and combine it through LDS.
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int j = blockIdx.x * 64 + lane;
+  const bool live = j < n && din[j] >= 0.0 && j != p;
+  double part = 0.0;
+  if (live) {
+    const double* Lj = L + j;
+    const double* Lp = L + p;
+    for (int l = wid; l < i; l += 4)
+      part += Lj[(int64_t)n * l] * Lp[(int64_t)n * l];
+  }
+  __shared__ double red[4][64];
+  red[wid][lane] = part;
+  __syncthreads();
+  if (wid != 0 || j >= n) return;
   const double dj = din[j];
   if (dj < 0.0) {
     dout[j] = dj;
@@ -261,10 +274,9 @@ pchol_step(const CholDesc* __restrict__ desc, const double* __restrict__ g_all,
     dout[j] = -1.0;
     return;
   }
-  double sm = G[j + (int64_t)n * p];
-  for (int l = 0; l < i; l++)
-    sm -= L[j + (int64_t)n * l] * L[p + (int64_t)n * l];
-  const double v = sm * inv;
+  const double sm = G[j + (int64_t)n * p] -
+                    ((red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]));
+  const double v = sm / rii;
   L[j + (int64_t)n * i] = v;
   dout[j] = fmax(dj - v * v, 0.0);
 }
@@ -337,8 +349,10 @@ void pchol_large(const std::vector<CholDesc>& h, const CholDesc* d_desc,
     maxn = std::max(maxn, x.n);
     maxsteps = std::max(maxsteps, std::min(x.kmax, x.n));
   }
-  const dim3 grid((maxn + 255) / 256, nb);
-  hipLaunchKernelGGL(pchol_init, grid, dim3(256), 0, s, d_desc, g, ws, state);
+  const dim3 grid_init((maxn + 255) / 256, nb);
+  const dim3 grid((maxn + 63) / 64, nb);
+  hipLaunchKernelGGL(pchol_init, grid_init, dim3(256), 0, s, d_desc, g, ws,
+                     state);
   std::vector<int> hst(4 * (size_t)nb);
   int i = 0;
   for (; i < maxsteps; i++) {

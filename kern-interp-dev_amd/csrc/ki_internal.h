@@ -230,6 +230,11 @@ template <typename T>
 void batched_lu(const std::vector<MatDesc>& h, MatDesc* d_desc, T* a,
                 int* piv, hipStream_t s);
 
+// X = U^{-1} B for upper-triangular U (no interchanges), blocked.
+template <typename T>
+void batched_trsm_upper(const std::vector<MatDesc>& h, MatDesc* d_desc,
+                        const T* a, T* b, hipStream_t s);
+
 // Solve with the LU factors: op(A) X = B, A = P L U (LAPACK getrs).
 template <typename T>
 void batched_getrs(const std::vector<MatDesc>& h, MatDesc* d_desc,

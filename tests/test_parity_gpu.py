@@ -6,6 +6,7 @@ oracle's (dgeqp3) sets; solutions within 1e-10 relative of the dense FP64
 solve after refinement.
 """
 import time
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -271,3 +272,48 @@ def test_c1_field_matches_exact_interior_field():
     err = np.abs(u - pb.exact).max() / np.abs(pb.exact).max()
     print("[c1] iters=%d relres=%.2e field err=%.2e" % (it, rel, err))
     assert err <= 1e-10
+
+
+@pytest.mark.parametrize("name", ["lap3d_sphere", "stokes3d_ellipsoid"])
+def test_gram_and_householder_ids_agree(name):
+    """The Gram form (pivoted Cholesky of A^T A) and the Householder CPQR
+    pick the same skeletons (same pivot rule) at tol >= 1e-7."""
+    b, pde, kw, leaf, prm, ctx, okern, otree = setup(name)
+    ctx.skeletonize(ki.default_params(precision=64, id_method=1, **kw))
+    hh = {(l, i): (ctx.box(l, i)["k"], set(ctx.box_list(l, i, "skel")))
+          for l in range(ctx.num_levels())
+          for i in range(ctx.level_size(l)) if ctx.box(l, i)["compressed"]}
+    ctx.skeletonize(ki.default_params(precision=64, id_method=2, **kw))
+    same = 0
+    for (l, i), (k, sk) in hh.items():
+        bx = ctx.box(l, i)
+        assert bx["compressed"] and bx["k"] == k, (l, i)
+        same += set(ctx.box_list(l, i, "skel")) == sk
+    assert same == len(hh)
+
+
+def test_wide_panel_lu_matches_batched_panel(monkeypatch):
+    """The top-level LU's multi-workgroup panel (used for tall blocks) gives
+    the same solution as the one-workgroup panel."""
+    b, pde, kw, leaf, prm, ctx, okern, otree = setup("stokes3d_ellipsoid")
+    rng = np.random.default_rng(8)
+    rhs = rng.normal(size=ctx.ndof)
+    ctx.skeletonize(ki.default_params(precision=64, **kw))
+    x1 = ctx.solve(rhs)
+    top = ctx.stats()["top_size"]
+    assert top > 256
+    import subprocess, sys, json, os
+    code = (
+        "import sys, json, numpy as np; sys.path.insert(0, %r);"
+        "sys.path.insert(0, %r); import test_parity_gpu as t;"
+        "b, pde, kw, leaf, prm, ctx, ok, ot = t.setup('stokes3d_ellipsoid');"
+        "ctx.skeletonize(t.ki.default_params(precision=64, **kw));"
+        "x = ctx.solve(np.random.default_rng(8).normal(size=ctx.ndof));"
+        "print(json.dumps(list(x)))" % (str(Path(__file__).parent.parent),
+                                         str(Path(__file__).parent)))
+    env = dict(os.environ, KI_WIDE_LU_ROWS="256")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True,
+                         text=True, env=env, timeout=300)
+    assert out.returncode == 0, out.stderr[-2000:]
+    x2 = np.array(json.loads(out.stdout.strip().splitlines()[-1]))
+    assert np.linalg.norm(x2 - x1) <= 1e-10 * np.linalg.norm(x1)
