@@ -140,6 +140,21 @@ int ki_proxy_block(ki_ctx* ctx, const ki_params* params, const double* center,
 int ki_apply_forward(ki_ctx* ctx, const double* dpts, int ndp,
                      const double* mu, double* out);
 
+/* Dense LU with partial pivoting of an n x n column-major FP64 matrix on the
+ * device: LU gets the packed factors, piv the 1-based row interchanges
+ * (LAPACK dgetrf convention); replaces ki_Mat::LU_factorize
+ * (ki_mat.cpp:445-453).  Large matrices take the multi-workgroup blocked
+ * path used for the top-level block. */
+int ki_lu_factorize(ki_ctx* ctx, const double* a, int n, double* lu,
+                    int* piv);
+
+/* Solve op(A) X = B with the factors from ki_lu_factorize (FP64, column-
+ * major B with nrhs columns, overwritten by X; trans = 1 solves with A^T);
+ * replaces ki_Mat::left_multiply_inverse / right_multiply_inverse (the
+ * dgetrs calls of ki_mat.cpp:487-513). */
+int ki_lu_solve(ki_ctx* ctx, const double* lu, const int* piv, int n,
+                double* b, int nrhs, int trans);
+
 /* Factorization statistics. */
 typedef struct ki_stats {
   int levels;

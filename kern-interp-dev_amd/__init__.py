@@ -32,6 +32,7 @@ EXPORTS = [
     "ki_tree_box_list", "ki_skeletonize", "ki_solve", "ki_matvec", "ki_gmres",
     "ki_set_rhs", "ki_gmres_resident", "ki_get_solution", "ki_kernel_block",
     "ki_apply_forward", "ki_get_stats", "ki_box_log", "ki_proxy_block",
+    "ki_lu_factorize", "ki_lu_solve",
 ]
 
 
@@ -121,6 +122,8 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "ki_box_log": (C.c_int, [vp, ip, C.c_int, ip]),
         "ki_proxy_block": (C.c_int, [vp, C.POINTER(KiParams), dp, C.c_double,
                                      ip, C.c_int, dp, ip]),
+        "ki_lu_factorize": (C.c_int, [vp, dp, C.c_int, dp, ip]),
+        "ki_lu_solve": (C.c_int, [vp, dp, ip, C.c_int, dp, C.c_int, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -306,6 +309,27 @@ class Context:
     def solution(self) -> np.ndarray:
         x = np.zeros(self.ndof)
         _check(self.lib.ki_get_solution(self.h, _dptr(x)))
+        return x
+
+    def lu_factorize(self, a):
+        """Device LU of a square matrix: (packed LU, 1-based pivots)."""
+        a = np.asfortranarray(a, dtype=np.float64)
+        n = a.shape[0]
+        lu = np.zeros((n, n), order="F")
+        piv = np.zeros(n, dtype=np.int32)
+        _check(self.lib.ki_lu_factorize(self.h, _dptr(a), n, _dptr(lu),
+                                        _iptr(piv)))
+        return lu, piv
+
+    def lu_solve(self, lu, piv, b, trans=False):
+        """Solve op(A) X = B with device factors (LAPACK getrs semantics)."""
+        lu = np.asfortranarray(lu, dtype=np.float64)
+        piv = np.ascontiguousarray(piv, dtype=np.int32)
+        x = np.asfortranarray(b, dtype=np.float64).copy(order="F")
+        n = lu.shape[0]
+        nrhs = 1 if x.ndim == 1 else x.shape[1]
+        _check(self.lib.ki_lu_solve(self.h, _dptr(lu), _iptr(piv), n,
+                                    _dptr(x), nrhs, int(bool(trans))))
         return x
 
     def stats(self) -> dict:

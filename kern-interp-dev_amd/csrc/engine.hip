@@ -1532,6 +1532,70 @@ int ki_apply_forward(ki_ctx* ctx, const double* dpts, int ndp,
   });
 }
 
+int ki_lu_factorize(ki_ctx* ctx, const double* a, int n, double* lu,
+                    int* piv) {
+  return guard([&] {
+    Ctx& c = ctx->c;
+    KI_CHECK(a && lu && piv && n > 0, KI_E_ARG, "bad LU arguments");
+    KI_HIP(hipSetDevice(c.device));
+    const size_t bytes = sizeof(double) * (size_t)n * n;
+    c.tmp.ensure(bytes);
+    c.qpiv.ensure(sizeof(int) * n);
+    KI_HIP(hipMemcpyAsync(c.tmp.p, a, bytes, hipMemcpyHostToDevice, c.s));
+    std::vector<ki::MatDesc> md(1);
+    md[0] = ki::MatDesc{};
+    md[0].m = md[0].n = n;
+    md[0].ld = n;
+    ki::upload(c.mdesc, md, c.s);
+    ki::batched_lu<double>(md, c.mdesc.as<ki::MatDesc>(), c.tmp.as<double>(),
+                           c.qpiv.as<int>(), c.s);
+    KI_HIP(hipMemcpyAsync(lu, c.tmp.p, bytes, hipMemcpyDeviceToHost, c.s));
+    KI_HIP(hipMemcpyAsync(piv, c.qpiv.p, sizeof(int) * n,
+                          hipMemcpyDeviceToHost, c.s));
+    KI_HIP(hipStreamSynchronize(c.s));
+    for (int i = 0; i < n; i++) piv[i] += 1;
+  });
+}
+
+int ki_lu_solve(ki_ctx* ctx, const double* lu, const int* piv, int n,
+                double* b, int nrhs, int trans) {
+  return guard([&] {
+    Ctx& c = ctx->c;
+    KI_CHECK(lu && piv && b && n > 0 && nrhs >= 0, KI_E_ARG,
+             "bad LU-solve arguments");
+    if (nrhs == 0) return;
+    KI_HIP(hipSetDevice(c.device));
+    const size_t abytes = sizeof(double) * (size_t)n * n;
+    const size_t bbytes = sizeof(double) * (size_t)n * nrhs;
+    c.tmp.ensure(abytes + bbytes);
+    c.qpiv.ensure(sizeof(int) * n);
+    std::vector<int> p0(piv, piv + n);
+    for (int i = 0; i < n; i++) {
+      KI_CHECK(p0[i] >= 1 && p0[i] <= n, KI_E_ARG, "pivot out of range");
+      p0[i] -= 1;
+    }
+    double* dA = c.tmp.as<double>();
+    KI_HIP(hipMemcpyAsync(dA, lu, abytes, hipMemcpyHostToDevice, c.s));
+    KI_HIP(hipMemcpyAsync(dA + (size_t)n * n, b, bbytes,
+                          hipMemcpyHostToDevice, c.s));
+    KI_HIP(hipMemcpyAsync(c.qpiv.p, p0.data(), sizeof(int) * n,
+                          hipMemcpyHostToDevice, c.s));
+    std::vector<ki::MatDesc> md(1);
+    md[0] = ki::MatDesc{};
+    md[0].m = md[0].n = n;
+    md[0].ld = n;
+    md[0].b_off = (int64_t)n * n;
+    md[0].ldb = n;
+    md[0].nrhs = nrhs;
+    ki::upload(c.mdesc, md, c.s);
+    ki::batched_getrs<double>(md, c.mdesc.as<ki::MatDesc>(), dA,
+                              c.qpiv.as<int>(), dA, trans != 0, c.s);
+    KI_HIP(hipMemcpyAsync(b, dA + (size_t)n * n, bbytes,
+                          hipMemcpyDeviceToHost, c.s));
+    KI_HIP(hipStreamSynchronize(c.s));
+  });
+}
+
 int ki_get_stats(ki_ctx* ctx, ki_stats* out) {
   return guard([&] {
     KI_CHECK(out != nullptr, KI_E_ARG, "null output");

@@ -13,16 +13,20 @@
 #include <cstdlib>
 
 #include "ki_internal.h"
+#include "mfma.h"
 
 namespace ki {
 
 namespace {
 
 // ---------------------------------------------------------------------------
-// GEMM: C = alpha op(A) op(B) + beta C, 64x64 output tile per workgroup,
-// 256 threads with a 4x4 register micro-tile, K staged through LDS.
+// GEMM: C = alpha op(A) op(B) + beta C on the matrix cores.  64x64 output
+// tile per workgroup, four waves of 32x32 (2x2 MFMA 16x16x4 tiles), K staged
+// through LDS in slices of 16.  The MFMA is fed B-slice-first so that the
+// accumulator's lane index runs along C's rows: column-major stores are
+// coalesced.
 // ---------------------------------------------------------------------------
-constexpr int GT = 64, GK = 16;
+constexpr int GT = 64, GK = 32;
 
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -37,14 +41,16 @@ gemm_kernel(const GemmDesc* __restrict__ desc, const T* __restrict__ a_all,
   const T* A = a_all + d.a_off;
   const T* B = b_all + d.b_off;
   T* C = c_all + d.c_off;
-  __shared__ T As[GK][GT + 1];
-  __shared__ T Bs[GK][GT + 1];
-  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-  T acc[4][4];
+  __shared__ T As[GK][GT + 4];
+  __shared__ T Bs[GK][GT + 4];
+  typedef typename Mfma<T>::acc_t acc_t;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int wr = (wid >> 1) * 32, wc = (wid & 1) * 32;
+  acc_t acc[2][2];
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+  for (int a = 0; a < 2; a++)
 #pragma unroll
-    for (int b = 0; b < 4; b++) acc[a][b] = 0;
+    for (int b = 0; b < 2; b++) acc[a][b] = acc_t{0, 0, 0, 0};
   for (int k0 = 0; k0 < d.k; k0 += GK) {
     // A tile: rows i0..i0+63 of op(A), cols k0..k0+15
     for (int e = threadIdx.x; e < GT * GK; e += 256) {
@@ -80,30 +86,36 @@ gemm_kernel(const GemmDesc* __restrict__ desc, const T* __restrict__ a_all,
     }
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < GK; kk++) {
-      T av[4], bv[4];
+    for (int ks = 0; ks < GK; ks += 4) {
+      const int kr = ks + (lane >> 4);
+      T av[2], bv[2];
 #pragma unroll
-      for (int a = 0; a < 4; a++) av[a] = As[kk][tx + 16 * a];
+      for (int a = 0; a < 2; a++) av[a] = As[kr][wr + 16 * a + (lane & 15)];
 #pragma unroll
-      for (int b = 0; b < 4; b++) bv[b] = Bs[kk][ty + 16 * b];
+      for (int b = 0; b < 2; b++) bv[b] = Bs[kr][wc + 16 * b + (lane & 15)];
 #pragma unroll
-      for (int a = 0; a < 4; a++)
+      for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] += av[a] * bv[b];
+        for (int b = 0; b < 2; b++)
+          acc[a][b] = Mfma<T>::mac(bv[b], av[a], acc[a][b]);
     }
     __syncthreads();
   }
+  // accumulator (a, b) element v: C row = lane % 16, C column = row(lane, v)
   const T alpha = (T)d.alpha, beta = (T)d.beta;
 #pragma unroll
-  for (int b = 0; b < 4; b++) {
-    const int gj = j0 + ty + 16 * b;
-    if (gj >= d.n) continue;
+  for (int a = 0; a < 2; a++) {
+    const int gi = i0 + wr + 16 * a + (lane & 15);
 #pragma unroll
-    for (int a = 0; a < 4; a++) {
-      const int gi = i0 + tx + 16 * a;
-      if (gi >= d.m) continue;
-      T* cp = C + gi + (int64_t)d.ldc * gj;
-      *cp = beta == (T)0 ? alpha * acc[a][b] : alpha * acc[a][b] + beta * *cp;
+    for (int b = 0; b < 2; b++) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int gj = j0 + wc + 16 * b + Mfma<T>::row(lane, v);
+        if (gi >= d.m || gj >= d.n) continue;
+        T* cp = C + gi + (int64_t)d.ldc * gj;
+        const T x = acc[a][b][v];
+        *cp = beta == (T)0 ? alpha * x : alpha * x + beta * *cp;
+      }
     }
   }
 }
@@ -271,6 +283,47 @@ lu_pstep(T* __restrict__ A, int ld, int m, int j0, int jb,
   }
 }
 
+// Row interchanges piv[r0..r1) applied, in order, to columns [clo, chi) of a
+// single matrix (one thread per column).
+template <typename T>
+__global__ void swap_cols_range(T* __restrict__ A, int ld,
+                                const int* __restrict__ piv, int r0, int r1,
+                                int clo, int chi) {
+  const int c = clo + blockIdx.x * 256 + threadIdx.x;
+  if (c >= chi) return;
+  T* cc = A + (int64_t)ld * c;
+  for (int r = r0; r < r1; r++) {
+    const int p = piv[r];
+    if (p != r) {
+      const T t = cc[r];
+      cc[r] = cc[p];
+      cc[p] = t;
+    }
+  }
+}
+
+// A[r0:r0+jb, c] <- L11^{-1} A[r0:r0+jb, c] for columns [clo, chi), L11 the
+// unit-lower jb x jb block at (r0, r0) staged in LDS.
+template <typename T>
+__global__ void __launch_bounds__(256)
+trsm_small_cols(T* __restrict__ A, int ld, int r0, int jb, int clo, int chi) {
+  __shared__ T Ls[32][33];
+  for (int e = threadIdx.x; e < 32 * 32; e += 256) {
+    const int t = e % 32, q = e / 32;
+    Ls[t][q] = (t < jb && q < jb) ? A[(r0 + t) + (int64_t)ld * (r0 + q)] : 0;
+  }
+  __syncthreads();
+  const int c = clo + blockIdx.x * 256 + threadIdx.x;
+  if (c >= chi) return;
+  // Plain loops: a fully unrolled register-array version of this solve gave
+  // wrong results on gfx950 (checked against LAPACK).
+  T* x = A + r0 + (int64_t)ld * c;
+  for (int q = 0; q < jb; q++) {
+    const T vq = x[q];
+    for (int t = q + 1; t < jb; t++) x[t] -= Ls[t][q] * vq;
+  }
+}
+
 template <typename T>
 __global__ void panel_copy_back(T* __restrict__ A, int ld, int m, int j0,
                                 int jb, const T* __restrict__ W) {
@@ -346,38 +399,32 @@ trsm_diag(const MatDesc* __restrict__ desc, const T* __restrict__ a_all,
   __syncthreads();
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= d.nrhs) return;
-  T* xp = b_all + d.b_off + (int64_t)d.ldb * col + r0;
-  T x[LB];
-#pragma unroll
-  for (int q = 0; q < LB; q++) x[q] = q < rb ? xp[q] : (T)0;
+  // Plain loops over the column in place (a fully unrolled register-array
+  // form of the same solve was miscomputed on gfx950).
+  T* x = b_all + d.b_off + (int64_t)d.ldb * col + r0;
   if (mode == 0) {  // unit lower
-#pragma unroll
-    for (int q = 0; q < LB; q++)
-#pragma unroll
-      for (int t = q + 1; t < LB; t++) x[t] -= D[t][q] * x[q];
+    for (int q = 0; q < rb; q++) {
+      const T v = x[q];
+      for (int t = q + 1; t < rb; t++) x[t] -= D[t][q] * v;
+    }
   } else if (mode == 1) {  // upper, non-unit
-#pragma unroll
-    for (int q = LB - 1; q >= 0; q--) {
-      if (q < rb) x[q] /= D[q][q];
-#pragma unroll
-      for (int t = 0; t < q; t++) x[t] -= D[t][q] * x[q];
+    for (int q = rb - 1; q >= 0; q--) {
+      const T v = x[q] / D[q][q];
+      x[q] = v;
+      for (int t = 0; t < q; t++) x[t] -= D[t][q] * v;
     }
   } else if (mode == 2) {  // U^T: lower, non-unit, (U^T)(t,q) = D[q][t]
-#pragma unroll
-    for (int q = 0; q < LB; q++) {
-      if (q < rb) x[q] /= D[q][q];
-#pragma unroll
-      for (int t = q + 1; t < LB; t++) x[t] -= D[q][t] * x[q];
+    for (int q = 0; q < rb; q++) {
+      const T v = x[q] / D[q][q];
+      x[q] = v;
+      for (int t = q + 1; t < rb; t++) x[t] -= D[q][t] * v;
     }
   } else {  // L^T: unit upper, (L^T)(t,q) = D[q][t]
-#pragma unroll
-    for (int q = LB - 1; q >= 0; q--)
-#pragma unroll
-      for (int t = 0; t < q; t++) x[t] -= D[q][t] * x[q];
+    for (int q = rb - 1; q >= 0; q--) {
+      const T v = x[q];
+      for (int t = 0; t < q; t++) x[t] -= D[q][t] * v;
+    }
   }
-#pragma unroll
-  for (int q = 0; q < LB; q++)
-    if (q < rb) xp[q] = x[q];
 }
 
 // Row interchanges on a right-hand-side block (laswp), forward or reverse.
@@ -473,9 +520,121 @@ void launch_gemm(const GemmDesc* d_desc, int ndesc, int max_m, int max_n,
 }
 
 template <typename T>
+static void trsm_sweeps(const std::vector<MatDesc>& h, MatDesc* d_desc,
+                        const T* a, const int* piv, T* b, const int* modes,
+                        int nmodes, int swap, hipStream_t s);
+
+// LU of one large matrix with two levels of blocking: 256-wide outer panels
+// whose trailing update is one K = 256 GEMM, each factored as 32-wide inner
+// panels (multi-workgroup column steps, interchanges and TRSM confined to the
+// outer panel, K = 32 GEMM inside it).  Same pivots as the one-level form.
+template <typename T>
+static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
+  constexpr int NB = 256;
+  const int m = x.m, n = x.n, ld = x.ld, mn = std::min(m, n);
+  T* A = a + x.off;
+  int* piv = piv_all + x.piv_off;
+  std::vector<GemmDesc> gd;
+  auto gemm_desc = [&](int r0, int c0, int k0, int kk, int rows, int cols) {
+    GemmDesc g{rows, cols, kk, x.off + r0 + (int64_t)ld * k0,
+               x.off + k0 + (int64_t)ld * c0, x.off + r0 + (int64_t)ld * c0,
+               ld, ld, ld, 0, 0, -1.0, 1.0};
+    gd.push_back(g);
+  };
+  // pass 1: descriptors in execution order
+  for (int J0 = 0; J0 < mn; J0 += NB) {
+    const int JB = std::min(NB, mn - J0);
+    for (int j0 = J0; j0 < J0 + JB; j0 += LB) {
+      const int jb = std::min(LB, J0 + JB - j0);
+      if (J0 + JB > j0 + jb)
+        gemm_desc(j0 + jb, j0 + jb, j0, jb, m - j0 - jb, J0 + JB - j0 - jb);
+    }
+    if (n > J0 + JB && m > J0 + JB)
+      gemm_desc(J0 + JB, J0 + JB, J0, JB, m - J0 - JB, n - J0 - JB);
+  }
+  GemmDesc* d_g = nullptr;
+  MatDesc* d_m = nullptr;
+  T* W = nullptr;
+  KI_HIP(hipMallocAsync((void**)&d_g, sizeof(GemmDesc) * std::max<size_t>(
+                                          gd.size(), 1), s));
+  if (!gd.empty())
+    KI_HIP(hipMemcpyAsync(d_g, gd.data(), sizeof(GemmDesc) * gd.size(),
+                          hipMemcpyHostToDevice, s));
+  KI_HIP(hipMallocAsync((void**)&d_m, sizeof(MatDesc), s));
+  KI_HIP(hipMallocAsync((void**)&W, sizeof(T) * (size_t)m * LB, s));
+  size_t gi = 0;
+  auto cols = [](int c) { return dim3((std::max(c, 1) + 255) / 256); };
+  for (int J0 = 0; J0 < mn; J0 += NB) {
+    const int JB = std::min(NB, mn - J0);
+    for (int j0 = J0; j0 < J0 + JB; j0 += LB) {
+      const int jb = std::min(LB, J0 + JB - j0);
+      const int mr = m - j0;
+      const dim3 g((mr + 255) / 256);
+      for (int c = 0; c < jb; c++)
+        hipLaunchKernelGGL(lu_pstep<T>, g, dim3(256), 0, s, A, ld, m, j0, jb,
+                           W, c, piv, c & 1);
+      if (jb & 1)
+        hipLaunchKernelGGL(panel_copy_back<T>, g, dim3(256), 0, s, A, ld, m,
+                           j0, jb, (const T*)W);
+      if (j0 > J0)
+        hipLaunchKernelGGL(swap_cols_range<T>, cols(j0 - J0), dim3(256), 0, s,
+                           A, ld, (const int*)piv, j0, j0 + jb, J0, j0);
+      if (J0 + JB > j0 + jb) {
+        const int c0 = j0 + jb, c1 = J0 + JB;
+        hipLaunchKernelGGL(swap_cols_range<T>, cols(c1 - c0), dim3(256), 0, s,
+                           A, ld, (const int*)piv, j0, j0 + jb, c0, c1);
+        hipLaunchKernelGGL(trsm_small_cols<T>, cols(c1 - c0), dim3(256), 0, s,
+                           A, ld, j0, jb, c0, c1);
+        const GemmDesc& q = gd[gi];
+        launch_gemm<T>(d_g + gi, 1, q.m, q.n, a, a, a, s);
+        gi++;
+      }
+    }
+    if (J0 > 0)
+      hipLaunchKernelGGL(swap_cols_range<T>, cols(J0), dim3(256), 0, s, A, ld,
+                         (const int*)piv, J0, J0 + JB, 0, J0);
+    if (n > J0 + JB) {
+      hipLaunchKernelGGL(swap_cols_range<T>, cols(n - J0 - JB), dim3(256), 0,
+                         s, A, ld, (const int*)piv, J0, J0 + JB, J0 + JB, n);
+      // U12 = L11^{-1} A12 (unit lower JB x JB), blocked
+      MatDesc md{};
+      md.m = md.n = JB;
+      md.off = x.off + J0 + (int64_t)ld * J0;
+      md.ld = ld;
+      md.b_off = x.off + J0 + (int64_t)ld * (J0 + JB);
+      md.ldb = ld;
+      md.nrhs = n - J0 - JB;
+      std::vector<MatDesc> hv{md};
+      KI_HIP(hipMemcpyAsync(d_m, &hv[0], sizeof(MatDesc),
+                            hipMemcpyHostToDevice, s));
+      const int mode0[1] = {0};
+      trsm_sweeps<T>(hv, d_m, a, nullptr, a, mode0, 1, 0, s);
+      if (m > J0 + JB) {
+        const GemmDesc& q = gd[gi];
+        launch_gemm<T>(d_g + gi, 1, q.m, q.n, a, a, a, s);
+        gi++;
+      }
+    }
+  }
+  KI_HIP(hipGetLastError());
+  KI_HIP(hipStreamSynchronize(s));
+  KI_HIP(hipFreeAsync(d_g, s));
+  KI_HIP(hipFreeAsync(d_m, s));
+  KI_HIP(hipFreeAsync(W, s));
+}
+
+template <typename T>
 void batched_lu(const std::vector<MatDesc>& h, MatDesc* d_desc, T* a, int* piv,
                 hipStream_t s) {
   if (h.empty()) return;
+  static const int wide_rows = [] {
+    const char* e = std::getenv("KI_WIDE_LU_ROWS");  // test override
+    return e ? std::atoi(e) : 4096;
+  }();
+  if (h.size() == 1 && h[0].m > wide_rows) {
+    lu_wide<T>(h[0], a, piv, s);
+    return;
+  }
   int maxmn = 0, maxn = 0;
   for (auto& x : h) {
     maxmn = std::max(maxmn, std::min(x.m, x.n));
@@ -514,34 +673,10 @@ void batched_lu(const std::vector<MatDesc>& h, MatDesc* d_desc, T* a, int* piv,
   KI_HIP(hipMallocAsync((void**)&d_g, sizeof(GemmDesc) * gd.size(), s));
   KI_HIP(hipMemcpyAsync(d_g, gd.data(), sizeof(GemmDesc) * gd.size(),
                         hipMemcpyHostToDevice, s));
-  // A single tall matrix (the top-level block) factors its panels with the
-  // multi-workgroup step kernel; batches use one workgroup per panel.
-  static const int wide_rows = [] {
-    const char* e = std::getenv("KI_WIDE_LU_ROWS");  // test override
-    return e ? std::atoi(e) : 4096;
-  }();
-  const bool wide = nb == 1 && h[0].m > wide_rows;
-  T* W = nullptr;
-  if (wide)
-    KI_HIP(hipMallocAsync((void**)&W, sizeof(T) * (size_t)h[0].m * LB, s));
   for (int p = 0; p < npan; p++) {
     const int j0 = p * LB;
-    if (wide) {
-      const MatDesc& x = h[0];
-      const int jb = std::min(LB, std::min(x.m, x.n) - j0);
-      const int mr = x.m - j0;
-      const dim3 g((mr + 255) / 256);
-      T* A0 = a + x.off;
-      for (int c = 0; c < jb; c++)
-        hipLaunchKernelGGL(lu_pstep<T>, g, dim3(256), 0, s, A0, x.ld, x.m, j0,
-                           jb, W, c, piv + x.piv_off, c & 1);
-      if (jb & 1)
-        hipLaunchKernelGGL(panel_copy_back<T>, g, dim3(256), 0, s, A0, x.ld,
-                           x.m, j0, jb, (const T*)W);
-    } else {
-      hipLaunchKernelGGL(lu_panel<T>, dim3(nb), dim3(256), 0, s, d_desc, a,
-                         piv, j0);
-    }
+    hipLaunchKernelGGL(lu_panel<T>, dim3(nb), dim3(256), 0, s, d_desc, a, piv,
+                       j0);
     hipLaunchKernelGGL(lu_swap_trsm<T>, dim3((maxn + 255) / 256, nb),
                        dim3(256), 0, s, d_desc, a, piv, j0);
     if (mm[p] > 0 && nn[p] > 0)
@@ -551,7 +686,6 @@ void batched_lu(const std::vector<MatDesc>& h, MatDesc* d_desc, T* a, int* piv,
   // the host descriptor vector dies here: make sure the upload has landed
   KI_HIP(hipStreamSynchronize(s));
   KI_HIP(hipFreeAsync(d_g, s));
-  if (W) KI_HIP(hipFreeAsync(W, s));
 }
 
 // Blocked triangular sweeps (modes as in trsm_diag) on every box, with the

@@ -20,12 +20,13 @@
 #include <cmath>
 
 #include "ki_internal.h"
+#include "mfma.h"
 
 namespace ki {
 
 namespace {
 
-constexpr int GT = 64, GK = 16;
+constexpr int GT = 64, GK = 32;
 
 template <typename TI>
 __global__ void __launch_bounds__(256)
@@ -42,14 +43,15 @@ gram_kernel(const GramDesc* __restrict__ desc, const TI* __restrict__ a_all,
   const int ti = t - tj * (tj + 1) / 2;
   const int i0 = ti * GT, j0 = tj * GT;
   const TI* A = a_all + d.a_off;
-  __shared__ double As[GK][GT + 1];
-  __shared__ double Bs[GK][GT + 1];
-  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-  double acc[4][4];
+  __shared__ double As[GK][GT + 4];
+  __shared__ double Bs[GK][GT + 4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int wr = (wid >> 1) * 32, wc = (wid & 1) * 32;
+  f64x4 acc[2][2];
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+  for (int a = 0; a < 2; a++)
 #pragma unroll
-    for (int b = 0; b < 4; b++) acc[a][b] = 0;
+    for (int b = 0; b < 2; b++) acc[a][b] = f64x4{0, 0, 0, 0};
   for (int k0 = 0; k0 < d.m; k0 += GK) {
     for (int e = threadIdx.x; e < GT * GK; e += 256) {
       const int kk = e % GK, ii = e / GK;
@@ -64,30 +66,35 @@ gram_kernel(const GramDesc* __restrict__ desc, const TI* __restrict__ a_all,
     }
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < GK; kk++) {
-      double av[4], bv[4];
+    for (int ks = 0; ks < GK; ks += 4) {
+      const int kr = ks + (lane >> 4);
+      double av[2], bv[2];
 #pragma unroll
-      for (int a = 0; a < 4; a++) av[a] = As[kk][tx + 16 * a];
+      for (int a = 0; a < 2; a++) av[a] = As[kr][wr + 16 * a + (lane & 15)];
 #pragma unroll
-      for (int b = 0; b < 4; b++) bv[b] = Bs[kk][ty + 16 * b];
+      for (int b = 0; b < 2; b++) bv[b] = Bs[kr][wc + 16 * b + (lane & 15)];
 #pragma unroll
-      for (int a = 0; a < 4; a++)
+      for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] += av[a] * bv[b];
+        for (int b = 0; b < 2; b++)
+          acc[a][b] = Mfma<double>::mac(bv[b], av[a], acc[a][b]);
     }
     __syncthreads();
   }
+  // accumulator (a, b) element v: G row = lane % 16, column = row(lane, v)
   double* G = g_all + d.g_off;
 #pragma unroll
-  for (int b = 0; b < 4; b++) {
-    const int gj = j0 + ty + 16 * b;
-    if (gj >= d.n) continue;
+  for (int a = 0; a < 2; a++) {
+    const int gi = i0 + wr + 16 * a + (lane & 15);
 #pragma unroll
-    for (int a = 0; a < 4; a++) {
-      const int gi = i0 + tx + 16 * a;
-      if (gi >= d.n) continue;
-      G[gi + (int64_t)d.n * gj] = acc[a][b];
-      G[gj + (int64_t)d.n * gi] = acc[a][b];
+    for (int b = 0; b < 2; b++) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int gj = j0 + wc + 16 * b + Mfma<double>::row(lane, v);
+        if (gi >= d.n || gj >= d.n) continue;
+        G[gi + (int64_t)d.n * gj] = acc[a][b][v];
+        G[gj + (int64_t)d.n * gi] = acc[a][b][v];
+      }
     }
   }
 }
@@ -211,13 +218,18 @@ pchol_wg(const CholDesc* __restrict__ desc, double* __restrict__ g_all,
   write_r(G, L, dg, piv, n, rank);
 }
 
-// One pivot step i for each large box (grid.y = box, grid.x = row chunks).
-// The Schur diagonal is double-buffered (din / dout) so that every
-// workgroup of a step reads the same values and picks the same pivot.
+// One pivot step i for each large box (grid.y = box, grid.x = row chunks),
+// blocked like LAPACK dpstrf: G holds the Schur complement S updated through
+// step i0 (a rank-CB GEMM after every CB steps), so a column of L needs only
+// S(:, p) and the block's own columns i0..i-1.  The Schur diagonal is
+// double-buffered (din / dout) so that every workgroup of a step reads the
+// same values and picks the same pivot.
+constexpr int CB = 32;
+
 __global__ void __launch_bounds__(256)
 pchol_step(const CholDesc* __restrict__ desc, const double* __restrict__ g_all,
            double* __restrict__ ws_all, int* __restrict__ piv_all,
-           int* __restrict__ state, int i, int parity, double tol) {
+           int* __restrict__ state, int i, int i0, int parity, double tol) {
   const CholDesc d = desc[blockIdx.y];
   const int n = d.n;
   int* st = state + 4 * d.slot;  // [rank, done, -, -]
@@ -249,7 +261,7 @@ pchol_step(const CholDesc* __restrict__ desc, const double* __restrict__ g_all,
     if (i == 0) ws_all[d.d_off + 2 * n] = thresh;
   }
   // This workgroup owns columns j0..j0+63 (one per lane); the four waves
-  // split the sum over l = 0..i-1 and combine it through LDS.
+  // split the sum over the block's columns l = i0..i-1 (LDS combine).
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int j = blockIdx.x * 64 + lane;
   const bool live = j < n && din[j] >= 0.0 && j != p;
@@ -257,7 +269,7 @@ pchol_step(const CholDesc* __restrict__ desc, const double* __restrict__ g_all,
   if (live) {
     const double* Lj = L + j;
     const double* Lp = L + p;
-    for (int l = wid; l < i; l += 4)
+    for (int l = i0 + wid; l < i; l += 4)
       part += Lj[(int64_t)n * l] * Lp[(int64_t)n * l];
   }
   __shared__ double red[4][64];
@@ -265,8 +277,9 @@ pchol_step(const CholDesc* __restrict__ desc, const double* __restrict__ g_all,
   __syncthreads();
   if (wid != 0 || j >= n) return;
   const double dj = din[j];
-  if (dj < 0.0) {
+  if (dj < 0.0) {  // selected earlier: zero keeps the block GEMM finite
     dout[j] = dj;
+    L[j + (int64_t)n * i] = 0.0;
     return;
   }
   if (j == p) {
@@ -353,11 +366,36 @@ void pchol_large(const std::vector<CholDesc>& h, const CholDesc* d_desc,
   const dim3 grid((maxn + 63) / 64, nb);
   hipLaunchKernelGGL(pchol_init, grid_init, dim3(256), 0, s, d_desc, g, ws,
                      state);
+  // Schur-complement updates S -= L(:, i0:i0+CB) L(:, i0:i0+CB)^T after each
+  // block of CB steps, for every box (descriptors uploaded once)
+  const int nblk = (maxsteps + CB - 1) / CB;
+  std::vector<GemmDesc> gd((size_t)nblk * nb);
+  for (int bk = 0; bk < nblk; bk++)
+    for (int b = 0; b < nb; b++) {
+      const CholDesc& x = h[b];
+      GemmDesc& q = gd[(size_t)bk * nb + b];
+      q = GemmDesc{0, 0, 0, 0, 0, 0, 1, 1, 1, 0, 1, -1.0, 1.0};
+      if ((bk + 1) * CB >= std::min(x.kmax, x.n)) continue;  // last block
+      q.m = q.n = x.n;
+      q.k = CB;
+      q.a_off = q.b_off = x.l_off + (int64_t)x.n * bk * CB;
+      q.lda = q.ldb = x.n;
+      q.c_off = x.g_off;
+      q.ldc = x.n;
+    }
+  GemmDesc* d_g = nullptr;
+  KI_HIP(hipMallocAsync((void**)&d_g, sizeof(GemmDesc) * gd.size(), s));
+  KI_HIP(hipMemcpyAsync(d_g, gd.data(), sizeof(GemmDesc) * gd.size(),
+                        hipMemcpyHostToDevice, s));
   std::vector<int> hst(4 * (size_t)nb);
   int i = 0;
   for (; i < maxsteps; i++) {
+    const int i0 = i - i % CB;
     hipLaunchKernelGGL(pchol_step, grid, dim3(256), 0, s, d_desc, g, ws, piv,
-                       state, i, i & 1, tol);
+                       state, i, i0, i & 1, tol);
+    if (i % CB == CB - 1 && i + 1 < maxsteps)
+      launch_gemm<double>(d_g + (size_t)(i / CB) * nb, nb, maxn, maxn, ws, ws,
+                          g, s);
     if ((i & 63) == 63) {  // stop early once every box has met its rank
       KI_HIP(hipMemcpyAsync(hst.data(), state, sizeof(int) * 4 * nb,
                             hipMemcpyDeviceToHost, s));
@@ -370,6 +408,8 @@ void pchol_large(const std::vector<CholDesc>& h, const CholDesc* d_desc,
   hipLaunchKernelGGL(pchol_finish, dim3(nb), dim3(256), 0, s, d_desc, g, ws,
                      piv, state, rank, i);
   KI_HIP(hipGetLastError());
+  KI_HIP(hipStreamSynchronize(s));  // descriptor vector dies with this scope
+  KI_HIP(hipFreeAsync(d_g, s));
 }
 
 template void launch_gram<float>(const GramDesc*, int, int, const float*,
