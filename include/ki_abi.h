@@ -167,8 +167,56 @@ typedef struct ki_stats {
   double t_solve_last;      /* seconds of the last factor apply              */
   double t_matvec_last;     /* seconds of the last FP64 matvec               */
   int64_t device_bytes;     /* device memory held by the factorization       */
+  int boxes_predicted;      /* boxes given a predicted rank                  */
+  int boxes_skipped;        /* predicted incompressible: not sent to the ID  */
 } ki_stats;
 int ki_get_stats(ki_ctx* ctx, ki_stats* out);
+
+/* Event-timed kernels of the last ki_skeletonize: *count is the number of
+ * entries; for 0 <= i < *count, name gets entry i's kernel name and
+ * vals[0..3] = launches (0: a kernel sequence), seconds, algorithmic flops,
+ * algorithmic bytes.  i < 0 only sets *count.  No reference counterpart
+ * (the reference reports wall-clock per phase only). */
+int ki_kernel_clock(ki_ctx* ctx, int i, char* name, int name_cap,
+                    double* vals, int* count);
+
+/* ---- Learned rank predictor (no reference counterpart) -------------------
+ * A 12-64-64-1 perceptron (tanh, tanh, sigmoid) predicting a box's ID rank
+ * fraction k / n from 12 features known before the ID.  Parameters are one
+ * flat FP32 vector of KI_MLP_NPARAM values (W1 [64][12], b1, W2 [64][64],
+ * b2, W3 [64], b3) held by the context.  With ki_params.use_predictor = 1
+ * and trained parameters, ki_skeletonize skips the ID of boxes predicted
+ * incompressible and stops every other box's pivoted factorization near its
+ * predicted rank; a box whose rank rule has not fired by then is redone in
+ * full (counted in ki_stats.boxes_fallback). */
+#define KI_MLP_NFEAT 12
+#define KI_MLP_NPARAM 5057
+
+/* Features (n x 12, row-major), labels (rank fraction; 1 when the rank rule
+ * never fired; -1 for boxes the predictor skipped) and the predictions used
+ * (-1 when none) of every box considered for compression by the last
+ * ki_skeletonize, in factorization order.  *count = number of boxes; copies
+ * min(cap, count).  Any output pointer may be NULL. */
+int ki_box_features(ki_ctx* ctx, float* feat, float* label, float* pred,
+                    int cap, int* count);
+/* Set / get the parameters (count must be KI_MLP_NPARAM). */
+int ki_mlp_set_params(ki_ctx* ctx, const float* params, int count);
+int ki_mlp_get_params(ki_ctx* ctx, float* params, int count);
+/* Initialize the parameters (Glorot-uniform weights, zero biases) from
+ * `seed`. */
+int ki_mlp_init(ki_ctx* ctx, uint64_t seed);
+/* Mean squared error of the predictions over (X, y) and its gradient with
+ * respect to the parameters at the current parameters (FP32, on the GPU). */
+int ki_mlp_grad(ki_ctx* ctx, const float* X, const float* y, int n,
+                float* grad, float* loss);
+/* Adam training on the GPU: `epochs` passes over the n samples in minibatches
+ * of `batch` (order shuffled per epoch from `seed`), learning rate lr.
+ * *loss = mean squared error over all samples after training. */
+int ki_mlp_train(ki_ctx* ctx, const float* X, const float* y, int n,
+                 int epochs, int batch, float lr, uint64_t seed, float* loss);
+/* Predicted rank fractions of n feature rows; bf16 = 1 runs the hidden
+ * layers on the BF16 matrix cores, 0 in FP32. */
+int ki_mlp_predict(ki_ctx* ctx, const float* X, int n, int bf16, float* out);
 
 /* Per compressed box: (level, n, m, k) quadruples in factorization order;
  * returns the number of boxes in *count, copies min(cap, count). */

@@ -32,8 +32,12 @@ EXPORTS = [
     "ki_tree_box_list", "ki_skeletonize", "ki_solve", "ki_matvec", "ki_gmres",
     "ki_set_rhs", "ki_gmres_resident", "ki_get_solution", "ki_kernel_block",
     "ki_apply_forward", "ki_get_stats", "ki_box_log", "ki_proxy_block",
-    "ki_lu_factorize", "ki_lu_solve",
+    "ki_lu_factorize", "ki_lu_solve", "ki_kernel_clock", "ki_box_features",
+    "ki_mlp_set_params", "ki_mlp_get_params", "ki_mlp_init", "ki_mlp_grad",
+    "ki_mlp_train", "ki_mlp_predict",
 ]
+
+MLP_NFEAT, MLP_NPARAM = 12, 5057
 
 
 class KiParams(C.Structure):
@@ -70,6 +74,8 @@ class KiStats(C.Structure):
         ("t_solve_last", C.c_double),
         ("t_matvec_last", C.c_double),
         ("device_bytes", C.c_int64),
+        ("boxes_predicted", C.c_int),
+        ("boxes_skipped", C.c_int),
     ]
 
     def as_dict(self):
@@ -91,6 +97,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     lib = C.CDLL(str(p))
     dp = C.POINTER(C.c_double)
     ip = C.POINTER(C.c_int)
+    fp = C.POINTER(C.c_float)
     vp = C.c_void_p
     sig = {
         "ki_params_default": (None, [C.POINTER(KiParams)]),
@@ -124,6 +131,16 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
                                      ip, C.c_int, dp, ip]),
         "ki_lu_factorize": (C.c_int, [vp, dp, C.c_int, dp, ip]),
         "ki_lu_solve": (C.c_int, [vp, dp, ip, C.c_int, dp, C.c_int, C.c_int]),
+        "ki_kernel_clock": (C.c_int, [vp, C.c_int, C.c_char_p, C.c_int, dp,
+                                      ip]),
+        "ki_box_features": (C.c_int, [vp, fp, fp, fp, C.c_int, ip]),
+        "ki_mlp_set_params": (C.c_int, [vp, fp, C.c_int]),
+        "ki_mlp_get_params": (C.c_int, [vp, fp, C.c_int]),
+        "ki_mlp_init": (C.c_int, [vp, C.c_uint64]),
+        "ki_mlp_grad": (C.c_int, [vp, fp, fp, C.c_int, fp, fp]),
+        "ki_mlp_train": (C.c_int, [vp, fp, fp, C.c_int, C.c_int, C.c_int,
+                                   C.c_float, C.c_uint64, fp]),
+        "ki_mlp_predict": (C.c_int, [vp, fp, C.c_int, C.c_int, fp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -149,6 +166,15 @@ def _dptr(a: np.ndarray):
 
 def _iptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _f32(a, shape=None) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a if shape is None else a.reshape(shape)
 
 
 def default_params(**kw) -> KiParams:
@@ -336,6 +362,72 @@ class Context:
         st = KiStats()
         _check(self.lib.ki_get_stats(self.h, C.byref(st)))
         return st.as_dict()
+
+    def kernel_clock(self) -> list:
+        """Event-timed kernels of the last skeletonize: dicts with name,
+        launches, seconds, flops, bytes."""
+        cnt = C.c_int()
+        _check(self.lib.ki_kernel_clock(self.h, -1, None, 0, None,
+                                        C.byref(cnt)))
+        out = []
+        for i in range(cnt.value):
+            name = C.create_string_buffer(128)
+            v = np.zeros(4)
+            _check(self.lib.ki_kernel_clock(self.h, i, name, 128, _dptr(v),
+                                            C.byref(cnt)))
+            out.append({"name": name.value.decode(), "launches": int(v[0]),
+                        "seconds": v[1], "flops": v[2], "bytes": v[3]})
+        return out
+
+    # -- learned rank predictor ---------------------------------------------
+    def box_features(self):
+        """(features n x 12, label n, prediction n) of the last skeletonize."""
+        cnt = C.c_int()
+        _check(self.lib.ki_box_features(self.h, None, None, None, 0,
+                                        C.byref(cnt)))
+        n = cnt.value
+        f = np.zeros((max(n, 1), MLP_NFEAT), dtype=np.float32)
+        y = np.zeros(max(n, 1), dtype=np.float32)
+        p = np.zeros(max(n, 1), dtype=np.float32)
+        _check(self.lib.ki_box_features(self.h, _fptr(f), _fptr(y), _fptr(p),
+                                        n, C.byref(cnt)))
+        return f[:n], y[:n], p[:n]
+
+    def mlp_init(self, seed: int = 0):
+        _check(self.lib.ki_mlp_init(self.h, seed))
+
+    def mlp_params(self) -> np.ndarray:
+        p = np.zeros(MLP_NPARAM, dtype=np.float32)
+        _check(self.lib.ki_mlp_get_params(self.h, _fptr(p), MLP_NPARAM))
+        return p
+
+    def mlp_set_params(self, p):
+        p = _f32(p)
+        _check(self.lib.ki_mlp_set_params(self.h, _fptr(p), p.size))
+
+    def mlp_grad(self, X, y):
+        X = _f32(X, (-1, MLP_NFEAT))
+        y = _f32(y)
+        g = np.zeros(MLP_NPARAM, dtype=np.float32)
+        loss = C.c_float()
+        _check(self.lib.ki_mlp_grad(self.h, _fptr(X), _fptr(y), len(y),
+                                    _fptr(g), C.byref(loss)))
+        return g, loss.value
+
+    def mlp_train(self, X, y, epochs=200, batch=256, lr=2e-3, seed=0):
+        X = _f32(X, (-1, MLP_NFEAT))
+        y = _f32(y)
+        loss = C.c_float()
+        _check(self.lib.ki_mlp_train(self.h, _fptr(X), _fptr(y), len(y),
+                                     epochs, batch, lr, seed, C.byref(loss)))
+        return loss.value
+
+    def mlp_predict(self, X, bf16=True) -> np.ndarray:
+        X = _f32(X, (-1, MLP_NFEAT))
+        out = np.zeros(len(X), dtype=np.float32)
+        _check(self.lib.ki_mlp_predict(self.h, _fptr(X), len(X), int(bf16),
+                                       _fptr(out)))
+        return out
 
     def box_log(self) -> np.ndarray:
         cnt = C.c_int()

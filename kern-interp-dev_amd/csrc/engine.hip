@@ -29,6 +29,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <random>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -84,8 +85,18 @@ struct LevelDev {
   DevBuf piv;       // X_rr pivots
   DevBuf idx;       // skel ++ redundant lists for the sweeps
   DevBuf sweep;     // SweepDesc array
-  int nsweep = 0, max_n = 0, max_r = 0;
+  DevBuf items;     // SweepItem lists of the four sweep phases
+  int item_off[4] = {0, 0, 0, 0}, item_cnt[4] = {0, 0, 0, 0};
+  int nsweep = 0, max_n = 0, max_r = 0, max_k = 0;
   int64_t nfac = 0;
+};
+
+// Event-timed totals of one kernel (or one fixed kernel sequence) over the
+// last skeletonize, with its algorithmic work.
+struct KClock {
+  std::string name;
+  int64_t launches = 0;
+  double seconds = 0, flops = 0, bytes = 0;
 };
 
 struct Ctx {
@@ -102,9 +113,13 @@ struct Ctx {
   bool factored = false;
   int prec = 64;
   std::vector<std::unique_ptr<LevelDev>> lev;
-  DevBuf facptrs, pivptrs;          // device arrays of per-level pointers
-  DevBuf diag_sweep;                // all compressed boxes (diag solve)
-  int ndiag = 0, diag_max_r = 0;
+  DevBuf facptrs;                   // device array of per-level pointers
+  // block-diagonal solve x[r] = X_rr^{-1} x[r] of every compressed box
+  DevBuf dg_desc, dg_gidx, dg_sidx, dg_work;
+  int dg_n = 0, dg_max_r = 0;
+  int64_t dg_total = 0;
+  // top-level solve plan (same kernels, one FP64 block)
+  DevBuf top_ddesc, top_gidx, top_sidx, top_ptr;
   // top level
   std::vector<int> allskel;
   DevBuf top, top_piv, d_allskel, top_vec, top_desc;
@@ -115,6 +130,25 @@ struct Ctx {
   DevBuf d_b, d_x, d_xt, d_r, d_w2, d_V, d_Z, d_red, d_tmpv, d_dp, d_dout;
   ki_stats stats{};
   std::vector<int> boxlog;
+  std::vector<KClock> kclk;
+  // learned rank predictor: parameters, Adam moments, scratch
+  DevBuf mlp_p, mlp_m, mlp_v, mlp_g, mlp_part, mlp_X, mlp_y, mlp_idx, mlp_out;
+  bool mlp_ready = false;
+  // per ID'd (or predictor-skipped) box of the last skeletonize: features,
+  // label (rank fraction; -1 when skipped) and the prediction used (-1: none)
+  std::vector<float> featlog, labellog, predlog;
+  void clock(const char* name, double sec, double flops, double bytes,
+             int64_t launches = 1) {
+    for (KClock& k : kclk)
+      if (k.name == name) {
+        k.launches += launches;
+        k.seconds += sec;
+        k.flops += flops;
+        k.bytes += bytes;
+        return;
+      }
+    kclk.push_back(KClock{name, launches, sec, flops, bytes});
+  }
   ~Ctx() {
     if (s) (void)hipStreamDestroy(s);
   }
@@ -216,6 +250,61 @@ void id_rows(const Ctx& c, const Box& bx, std::vector<int>& rows,
   }
 }
 
+// Learned-rank use: boxes predicted at or above this rank fraction are not
+// sent to the ID (their DOFs pass to the parent, which is always exact);
+// the others stop their pivoted factorization at margin * predicted rank
+// + slack steps, and a box whose rank rule has not fired by then is redone
+// with the full factorization.
+constexpr float kSkipFraction = 0.97f;
+constexpr double kRankMargin = 1.15;
+constexpr int kRankSlack = 16;
+
+// The 12 features of a box, known before its ID:
+//   0 level / 8                     1 log2(n active DOFs) / 12
+//   2 log2(original DOFs + 1) / 12  3 log2(ID rows m) / 16
+//   4 -log10(tol) / 10              5 solution dimension / 3
+//   6 domain dimension / 3          7-9 extents of the active points'
+//   bounding box / box side, descending (0 in 2D for the third)
+//   10 1 - |mean unit normal| of the active points
+//   11 1 when the ID rows follow the level-1 / level-2 rule (no proxy)
+void box_features(const Ctx& c, const Box& bx, int m, bool special,
+                  double tol, float* f) {
+  const int n = bx.active.size();
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, ns[3] = {0, 0, 0};
+  int np = 0, last = -1;
+  for (int dof : bx.active) {
+    const int p = dof / c.sol;
+    if (p == last) continue;
+    last = p;
+    for (int a = 0; a < c.dom; a++) {
+      const double x = c.pts[(int64_t)p * c.dom + a];
+      if (np == 0 || x < lo[a]) lo[a] = x;
+      if (np == 0 || x > hi[a]) hi[a] = x;
+      ns[a] += c.nrm[(int64_t)p * c.dom + a];
+    }
+    np++;
+  }
+  double ext[3] = {0, 0, 0};
+  for (int a = 0; a < c.dom; a++)
+    ext[a] = bx.side > 0 ? (hi[a] - lo[a]) / bx.side : 0.0;
+  std::sort(ext, ext + 3, [](double x, double y) { return x > y; });
+  const double mn =
+      np > 0 ? std::sqrt(ns[0] * ns[0] + ns[1] * ns[1] + ns[2] * ns[2]) / np
+             : 1.0;
+  f[0] = bx.level / 8.0f;
+  f[1] = (float)(std::log2((double)std::max(n, 1)) / 12.0);
+  f[2] = (float)(std::log2((double)bx.original.size() + 1.0) / 12.0);
+  f[3] = (float)(std::log2((double)std::max(m, 1)) / 16.0);
+  f[4] = (float)(-std::log10(tol) / 10.0);
+  f[5] = c.sol / 3.0f;
+  f[6] = c.dom / 3.0f;
+  f[7] = (float)ext[0];
+  f[8] = (float)ext[1];
+  f[9] = (float)ext[2];
+  f[10] = (float)(1.0 - mn);
+  f[11] = special ? 1.0f : 0.0f;
+}
+
 struct Timer {
   hipEvent_t a, b;
   hipStream_t s;
@@ -256,6 +345,10 @@ void skeletonize_t(Ctx& c) {
   c.lev.clear();
   for (int l = 0; l < L; l++) c.lev.emplace_back(new LevelDev());
   c.boxlog.clear();
+  c.kclk.clear();
+  c.featlog.clear();
+  c.labellog.clear();
+  c.predlog.clear();
   ki_stats& st = c.stats;
   st = ki_stats{};
   st.levels = L;
@@ -296,24 +389,70 @@ void skeletonize_t(Ctx& c) {
       std::fflush(stderr);
     }};
     tr.remove_inactive_dofs_at_level(level);
-    // ---- ID matrices -------------------------------------------------
-    std::vector<int> cand, idx;
-    std::vector<FillDesc> fd;
-    std::vector<QrDesc> qd;
-    int64_t idm_el = 0, ws_el = 0, piv_el = 0;
-    int max_cols = 0, max_n = 0;
-    std::vector<int> rows;
+    // ---- candidates, features, predicted ranks -------------------------
+    struct Cand {
+      int b, m;
+      bool special;
+      std::vector<int> rows;
+    };
+    std::vector<Cand> cands;
     for (int b : tr.levels[level]) {
       Box& bx = tr.boxes[b];
       if (bx.compressed ||
           (int)bx.active.size() < c.prm.min_dofs_to_compress)
         continue;
-      bool special = false;
-      id_rows(c, bx, rows, special);
+      Cand cd{b, 0, false, {}};
+      id_rows(c, bx, cd.rows, cd.special);
+      cd.m = 2 * (int)cd.rows.size() + (cd.special ? 0 : P);
+      if (cd.m == 0) continue;
+      cands.push_back(std::move(cd));
+    }
+    const int ncand = cands.size();
+    std::vector<float> feat((size_t)ncand * MLP_NF), pred(ncand, -1.0f);
+    for (int q = 0; q < ncand; q++)
+      box_features(c, tr.boxes[cands[q].b], cands[q].m, cands[q].special, tol,
+                   feat.data() + (size_t)q * MLP_NF);
+    const bool predict = c.prm.use_predictor && c.mlp_ready && ncand > 0;
+    if (predict) {
+      upload(c.mlp_X, feat, s);
+      c.mlp_out.ensure(sizeof(float) * ncand);
+      mlp_infer(c.mlp_p.as<float>(), c.mlp_X.as<float>(), ncand,
+                c.mlp_out.as<float>(), 1, s);
+      KI_HIP(hipMemcpyAsync(pred.data(), c.mlp_out.p, sizeof(float) * ncand,
+                            hipMemcpyDeviceToHost, s));
+      KI_HIP(hipStreamSynchronize(s));
+    }
+    // ---- ID matrices -------------------------------------------------
+    std::vector<int> cand, idx;
+    std::vector<FillDesc> fd;
+    std::vector<QrDesc> qd;
+    std::vector<int> cand_feat;  // candidate slot of each ID'd box
+    int64_t idm_el = 0, ws_el = 0, piv_el = 0;
+    int max_cols = 0, max_n = 0;
+    for (int qc = 0; qc < ncand; qc++) {
+      const Cand& cd = cands[qc];
+      const int b = cd.b;
+      Box& bx = tr.boxes[b];
+      const std::vector<int>& rows = cd.rows;
+      const bool special = cd.special;
       const int n = bx.active.size();
       const int nr = rows.size();
-      const int m = 2 * nr + (special ? 0 : P);
-      if (m == 0) continue;
+      const int m = cd.m;
+      int kcap = n;
+      if (predict) {
+        st.boxes_predicted++;
+        // predicted incompressible: keep the box's DOFs for its parent
+        if (pred[qc] >= kSkipFraction) {
+          st.boxes_skipped++;
+          c.featlog.insert(c.featlog.end(), feat.begin() + (size_t)qc * MLP_NF,
+                           feat.begin() + (size_t)(qc + 1) * MLP_NF);
+          c.labellog.push_back(-1.0f);
+          c.predlog.push_back(pred[qc]);
+          continue;
+        }
+        kcap = std::min(n, (int)std::ceil(kRankMargin * pred[qc] * n) +
+                               kRankSlack);
+      }
       FillDesc f{};
       f.ncols = n;
       f.col_idx = idx.size();
@@ -345,7 +484,7 @@ void skeletonize_t(Ctx& c) {
       q.a_off = idm_el;
       q.piv_off = piv_el;
       q.ws_off = ws_el;
-      q.max_rank = n;
+      q.max_rank = kcap;
       qd.push_back(q);
       bx.id_rows = m;
       idm_el += (int64_t)m * n;
@@ -354,6 +493,7 @@ void skeletonize_t(Ctx& c) {
       max_cols = std::max(max_cols, n);
       max_n = std::max(max_n, n);
       cand.push_back(b);
+      cand_feat.push_back(qc);
     }
     const int nc = cand.size();
     lv_nc = nc;
@@ -364,49 +504,94 @@ void skeletonize_t(Ctx& c) {
       lv_maxm = std::max(lv_maxm, q.m);
     }
     std::vector<int> rank(nc, 0), hpiv(piv_el);
+    gofs.assign(nc, 0);
     if (nc > 0) {
       c.idm.ensure(sizeof(T) * idm_el);
       c.qws.ensure(sizeof(T) * ws_el);
       c.qpiv.ensure(sizeof(int) * piv_el);
       c.qrank.ensure(sizeof(int) * nc);
       upload(c.idxb, idx, s);
-      upload(c.fdesc, fd, s);
-      upload(c.qdesc, qd, s);
-      {
-        Timer tm(s);
-        launch_fill<T>(c.g, px, c.fdesc.as<FillDesc>(), nc, max_cols,
-                       c.idxb.as<int>(), c.idm.as<T>(), s);
-        st.t_fill += tm.stop();
-      }
-      if (!use_gram) {
-        Timer tm(s);
-        launch_cpqr<T>(c.qdesc.as<QrDesc>(), nc, max_n, c.idm.as<T>(),
-                       c.qpiv.as<int>(), c.qws.as<T>(), c.qrank.as<int>(),
-                       tol, s);
-        st.t_qr += tm.stop();
-      } else {
-        // G = A^T A (FP64), pivoted Cholesky, R over G, T = R11^-1 R12
-        Timer tm(s);
-        std::vector<GramDesc> gd(nc);
-        std::vector<CholDesc> small, large;
-        int64_t g_el = 0, w_el = 0;
-        gofs.assign(nc, 0);
+      // Gram / Cholesky storage for every ID'd box (offsets fixed up front)
+      int64_t g_el = 0, w_el = 0;
+      std::vector<int64_t> wof(nc, 0);
+      if (use_gram) {
         for (int q = 0; q < nc; q++) {
-          const int n = qd[q].n;
-          gd[q] = GramDesc{qd[q].m, n, qd[q].a_off, qd[q].m, g_el};
-          CholDesc ch{n, n, q, g_el, w_el, w_el + (int64_t)n * n,
-                      qd[q].piv_off};
-          (n > 640 ? large : small).push_back(ch);
+          const int64_t n = qd[q].n;
           gofs[q] = g_el;
-          g_el += (int64_t)n * n;
-          w_el += (int64_t)n * n + 2 * n + 1;
+          wof[q] = w_el;
+          g_el += n * n;
+          w_el += n * n + 2 * n + 1;
         }
         c.gram.ensure(sizeof(double) * g_el);
         c.cws.ensure(sizeof(double) * w_el);
         c.cstate.ensure(sizeof(int) * 4 * (nc + 1));
+      }
+      // fill + ID of the boxes `sel` (all boxes, then predictor fallbacks)
+      auto run_id = [&](const std::vector<int>& sel) {
+        std::vector<FillDesc> sf;
+        std::vector<QrDesc> sq;
+        int mc = 0;
+        double el = 0.;
+        for (int q : sel) {
+          sf.push_back(fd[q]);
+          sq.push_back(qd[q]);
+          mc = std::max(mc, qd[q].n);
+          el += (double)qd[q].m * qd[q].n;
+        }
+        const int ns = sel.size();
+        upload(c.fdesc, sf, s);
+        {
+          Timer tm(s);
+          launch_fill<T>(c.g, px, c.fdesc.as<FillDesc>(), ns, mc,
+                         c.idxb.as<int>(), c.idm.as<T>(), s);
+          const double dt = tm.stop();
+          st.t_fill += dt;
+          c.clock("fill_kernel (ID matrices)", dt, 0., sizeof(T) * el);
+        }
+        if (!use_gram) {
+          // the CPQR writes the rank of descriptor i to slot i
+          upload(c.qdesc, sq, s);
+          Timer tm(s);
+          launch_cpqr<T>(c.qdesc.as<QrDesc>(), ns, mc, c.idm.as<T>(),
+                         c.qpiv.as<int>(), c.qws.as<T>(), c.qrank.as<int>(),
+                         tol, s);
+          const double dt = tm.stop();
+          st.t_qr += dt;
+          // algorithmic bytes: the ID matrices read once
+          c.clock("cpqr_wg", dt, 0., sizeof(T) * el);
+          std::vector<int> r(ns);
+          KI_HIP(hipMemcpyAsync(r.data(), c.qrank.p, sizeof(int) * ns,
+                                hipMemcpyDeviceToHost, s));
+          KI_HIP(hipStreamSynchronize(s));
+          for (int i = 0; i < ns; i++) rank[sel[i]] = r[i];
+          return;
+        }
+        // G = A^T A (FP64), pivoted Cholesky, R over G, T = R11^-1 R12
+        Timer tm(s);
+        std::vector<GramDesc> gd;
+        std::vector<CholDesc> small, large;
+        for (int q : sel) {
+          const int n = qd[q].n;
+          gd.push_back(GramDesc{qd[q].m, n, qd[q].a_off, qd[q].m, gofs[q]});
+          CholDesc ch{n,       qd[q].max_rank, q, gofs[q], wof[q],
+                      wof[q] + (int64_t)n * n, qd[q].piv_off};
+          (n > 640 ? large : small).push_back(ch);
+        }
         upload(c.gdesc, gd, s);
-        launch_gram<T>(c.gdesc.as<GramDesc>(), nc, max_n, c.idm.as<T>(),
-                       c.gram.as<double>(), s);
+        {
+          // G = A^T A: m n (n + 1) flops (symmetric rank-m update), A read
+          // once and G written once.
+          double fl = 0., by = 0.;
+          for (const GramDesc& g : gd) {
+            fl += (double)g.m * g.n * (g.n + 1.0);
+            by += sizeof(T) * (double)g.m * g.n + 8.0 * g.n * g.n;
+          }
+          Timer tg(s);
+          launch_gram<T>(c.gdesc.as<GramDesc>(), ns, mc, c.idm.as<T>(),
+                         c.gram.as<double>(), s);
+          c.clock("gram_kernel", tg.stop(), fl, by);
+        }
+        Timer tch(s);
         if (!small.empty()) {
           upload(c.cdesc, small, s);
           pchol_small(c.cdesc.as<CholDesc>(), small.size(),
@@ -420,13 +605,41 @@ void skeletonize_t(Ctx& c) {
                       c.cws.as<double>(), c.qpiv.as<int>(),
                       c.cstate.as<int>(), c.qrank.as<int>(), tol, s);
         }
+        c.clock("pivoted Cholesky (pchol_wg, pchol_step)", tch.stop(), 0., 0.,
+                0);
         st.t_qr += tm.stop();
+        std::vector<int> r(nc);
+        KI_HIP(hipMemcpyAsync(r.data(), c.qrank.p, sizeof(int) * nc,
+                              hipMemcpyDeviceToHost, s));
+        KI_HIP(hipStreamSynchronize(s));
+        for (int q : sel) rank[q] = r[q];
+      };
+      std::vector<int> all(nc);
+      for (int q = 0; q < nc; q++) all[q] = q;
+      run_id(all);
+      // A predicted rank cap that the rank rule did not reach: redo the box
+      // with the full pivoted factorization.
+      std::vector<int> redo;
+      for (int q = 0; q < nc; q++)
+        if (rank[q] == 0 && qd[q].max_rank < qd[q].n) {
+          qd[q].max_rank = qd[q].n;
+          redo.push_back(q);
+        }
+      if (!redo.empty()) {
+        st.boxes_fallback += redo.size();
+        run_id(redo);
       }
-      KI_HIP(hipMemcpyAsync(rank.data(), c.qrank.p, sizeof(int) * nc,
-                            hipMemcpyDeviceToHost, s));
       KI_HIP(hipMemcpyAsync(hpiv.data(), c.qpiv.p, sizeof(int) * piv_el,
                             hipMemcpyDeviceToHost, s));
       KI_HIP(hipStreamSynchronize(s));
+      for (int q = 0; q < nc; q++) {
+        // feature log: label = rank fraction (1 when the rule never fired)
+        const int qc = cand_feat[q];
+        c.featlog.insert(c.featlog.end(), feat.begin() + (size_t)qc * MLP_NF,
+                         feat.begin() + (size_t)(qc + 1) * MLP_NF);
+        c.labellog.push_back(rank[q] > 0 ? (float)rank[q] / qd[q].n : 1.0f);
+        c.predlog.push_back(pred[qc]);
+      }
       for (int q = 0; q < nc; q++) {
         const int64_t mn = (int64_t)qd[q].m * qd[q].n;
         st.bytes_fill += sizeof(T) * (double)mn;
@@ -875,28 +1088,69 @@ void skeletonize_t(Ctx& c) {
     md[0].b_off = 0;
     md[0].ldb = nt;
     upload(c.top_desc, md, s);
-    batched_lu<double>(md, c.top_desc.as<MatDesc>(), c.top.as<double>(),
-                       c.top_piv.as<int>(), s);
     st.flops_top = 2.0 / 3.0 * (double)nt * nt * nt;
+    {
+      Timer tl(s);
+      batched_lu<double>(md, c.top_desc.as<MatDesc>(), c.top.as<double>(),
+                         c.top_piv.as<int>(), s);
+      c.clock("top-level LU (FP64)", tl.stop(), st.flops_top,
+              16.0 * (double)nt * nt, 0);
+    }
     st.t_top = ttop.stop();
   }
 
   // ---- solve plans -------------------------------------------------------
+  // Row interchanges of an LU, applied in order to 0..n-1, give the gather
+  // permutation of getrs: (P b)[j] = b[perm[j]].
+  auto compose = [](const int* piv, int n, std::vector<int>& perm) {
+    perm.resize(n);
+    for (int j = 0; j < n; j++) perm[j] = j;
+    for (int q = 0; q < n; q++)
+      if (piv[q] != q) std::swap(perm[q], perm[piv[q]]);
+  };
   std::vector<T*> fptr(L, nullptr);
-  std::vector<int*> pptr(L, nullptr);
-  std::vector<SweepDesc> all;
+  std::vector<DiagDesc> dd;
+  std::vector<int> dgi, dsi, perm, hp;
   int64_t dev_bytes = 0;
+  c.dg_max_r = 0;
   for (int l = 0; l < L; l++) {
     LevelDev& LD = *c.lev[l];
     fptr[l] = LD.fac.as<T>();
-    pptr[l] = LD.piv.as<int>();
     dev_bytes += LD.fac.bytes + LD.piv.bytes;
     std::vector<SweepDesc> sw;
+    std::vector<SweepItem> items[4];
     std::vector<int> li;
-    LD.max_n = LD.max_r = 0;
+    LD.max_n = LD.max_r = LD.max_k = 0;
+    hp.assign(LD.piv.bytes / sizeof(int), 0);
+    if (!hp.empty()) {
+      KI_HIP(hipMemcpyAsync(hp.data(), LD.piv.p, LD.piv.bytes,
+                            hipMemcpyDeviceToHost, s));
+      KI_HIP(hipStreamSynchronize(s));
+    }
     for (int b : tr.levels[l]) {
       const Box& bx = tr.boxes[b];
       if (!bx.compressed) continue;
+      const int di = sw.size();
+      for (int ph = 0; ph < 4; ph++) {
+        const int nout = (ph == 0 || ph == 2) ? bx.r : bx.k;
+        const int step = sweep_items_per_item(ph);
+        for (int o0 = 0; o0 < nout; o0 += step)
+          items[ph].push_back(SweepItem{di, o0});
+      }
+      if (bx.r > 0) {
+        compose(hp.data() + bx.offPiv, bx.r, perm);
+        DiagDesc g{};
+        g.r = bx.r;
+        g.buf = l;
+        g.offX = bx.offX;
+        g.w_off = dgi.size();
+        dd.push_back(g);
+        for (int j = 0; j < bx.r; j++) {
+          dgi.push_back(bx.redundant[perm[j]]);
+          dsi.push_back(bx.redundant[j]);
+        }
+        c.dg_max_r = std::max(c.dg_max_r, bx.r);
+      }
       SweepDesc x{};
       x.k = bx.k;
       x.r = bx.r;
@@ -913,17 +1167,48 @@ void skeletonize_t(Ctx& c) {
       sw.push_back(x);
       LD.max_n = std::max(LD.max_n, bx.k + bx.r);
       LD.max_r = std::max(LD.max_r, bx.r);
+      LD.max_k = std::max(LD.max_k, bx.k);
     }
     LD.nsweep = sw.size();
     KI_CHECK((int64_t)LD.max_n * (int64_t)sizeof(T) <= 64 * 1024, KI_E_ARG,
              "box too large for the LDS-staged sweep kernel");
     if (!sw.empty()) {
+      std::vector<SweepItem> flat;
+      for (int ph = 0; ph < 4; ph++) {
+        LD.item_off[ph] = flat.size();
+        LD.item_cnt[ph] = items[ph].size();
+        flat.insert(flat.end(), items[ph].begin(), items[ph].end());
+      }
       upload(LD.idx, li, s);
       upload(LD.sweep, sw, s);
+      upload(LD.items, flat, s);
     }
   }
   upload(c.facptrs, fptr, s);
-  upload(c.pivptrs, pptr, s);
+  c.dg_n = dd.size();
+  c.dg_total = dgi.size();
+  if (c.dg_n > 0) {
+    upload(c.dg_desc, dd, s);
+    upload(c.dg_gidx, dgi, s);
+    upload(c.dg_sidx, dsi, s);
+    c.dg_work.ensure(sizeof(T) * c.dg_total);
+  }
+  if (nt > 0) {
+    std::vector<int> tp(nt);
+    KI_HIP(hipMemcpyAsync(tp.data(), c.top_piv.p, sizeof(int) * nt,
+                          hipMemcpyDeviceToHost, s));
+    KI_HIP(hipStreamSynchronize(s));
+    compose(tp.data(), nt, perm);
+    std::vector<int> tg(nt);
+    for (int j = 0; j < nt; j++) tg[j] = c.allskel[perm[j]];
+    upload(c.top_gidx, tg, s);
+    upload(c.top_sidx, c.allskel, s);
+    std::vector<DiagDesc> td{DiagDesc{nt, 0, 0, 0}};
+    upload(c.top_ddesc, td, s);
+    std::vector<double*> tptr{c.top.as<double>()};
+    upload(c.top_ptr, tptr, s);
+    c.top_vec.ensure(sizeof(double) * nt);
+  }
   KI_HIP(hipStreamSynchronize(s));
   st.device_bytes = dev_bytes + c.top.bytes;
   st.t_total = now() - t_start;
@@ -939,39 +1224,37 @@ void solve_t(Ctx& c, const double* b_in, double* x_out) {
   T* xt = c.d_xt.as<T>();
   launch_convert<T, double>(b_in, xt, c.ndof, s);
   T* const* fb = c.facptrs.as<T*>();
-  int* const* pb = c.pivptrs.as<int*>();
+  auto phase = [&](LevelDev& LD, int ph) {
+    const int len = (ph == 0 || ph == 2) ? LD.max_k : LD.max_r;
+    launch_sweep_phase<T>(LD.sweep.as<SweepDesc>(),
+                          LD.items.as<SweepItem>() + LD.item_off[ph],
+                          LD.item_cnt[ph], LD.idx.as<int>(), fb, xt, ph, len,
+                          s);
+  };
+  // up-sweeps: x[r] -= T^T x[s]; x[s] -= L x[r]  (levels L-1 .. 0)
   for (int l = L - 1; l >= 0; l--) {
     LevelDev& LD = *c.lev[l];
     if (LD.nsweep == 0) continue;
-    launch_sweep<T>(LD.sweep.as<SweepDesc>(), LD.nsweep, LD.idx.as<int>(), fb,
-                    xt, 1, s, LD.max_n);
+    phase(LD, 0);
+    phase(LD, 1);
   }
-  for (int l = L - 1; l >= 0; l--) {
-    LevelDev& LD = *c.lev[l];
-    if (LD.nsweep == 0) continue;
-    launch_diag_solve<T>(LD.sweep.as<SweepDesc>(), LD.nsweep, LD.idx.as<int>(),
-                         fb, pb, xt, s, std::max(LD.max_r, 1));
-  }
+  // x[r] = X_rr^{-1} x[r] for every compressed box
+  launch_diag_blocked<T, T>(c.dg_desc.as<DiagDesc>(), c.dg_n, c.dg_max_r,
+                            c.dg_gidx.as<int>(), c.dg_sidx.as<int>(),
+                            c.dg_total, fb, xt, c.dg_work.as<T>(), s);
+  // top level, FP64
   const int nt = c.allskel.size();
-  if (nt > 0) {
-    c.top_vec.ensure(sizeof(double) * nt);
-    double* tv = c.top_vec.as<double>();
-    launch_gather<T, double>(xt, c.d_allskel.as<int>(), nt, tv, s);
-    std::vector<MatDesc> md(1);
-    md[0] = MatDesc{};
-    md[0].m = md[0].n = nt;
-    md[0].ld = nt;
-    md[0].nrhs = 1;
-    md[0].ldb = nt;
-    batched_getrs<double>(md, c.top_desc.as<MatDesc>(), c.top.as<double>(),
-                          c.top_piv.as<int>(), tv, false, s);
-    launch_scatter_vec<double, T>(tv, c.d_allskel.as<int>(), nt, xt, s);
-  }
+  if (nt > 0)
+    launch_diag_blocked<T, double>(
+        c.top_ddesc.as<DiagDesc>(), 1, nt, c.top_gidx.as<int>(),
+        c.top_sidx.as<int>(), nt, c.top_ptr.as<double*>(), xt,
+        c.top_vec.as<double>(), s);
+  // down-sweeps: x[r] -= U x[s]; x[s] -= T x[r]  (levels 0 .. L-1)
   for (int l = 0; l < L; l++) {
     LevelDev& LD = *c.lev[l];
     if (LD.nsweep == 0) continue;
-    launch_sweep<T>(LD.sweep.as<SweepDesc>(), LD.nsweep, LD.idx.as<int>(), fb,
-                    xt, 0, s, LD.max_n);
+    phase(LD, 2);
+    phase(LD, 3);
   }
   launch_convert<double, T>(xt, x_out, c.ndof, s);
 }
@@ -1600,6 +1883,177 @@ int ki_get_stats(ki_ctx* ctx, ki_stats* out) {
   return guard([&] {
     KI_CHECK(out != nullptr, KI_E_ARG, "null output");
     *out = ctx->c.stats;
+  });
+}
+
+int ki_box_features(ki_ctx* ctx, float* feat, float* label, float* pred,
+                    int cap, int* count) {
+  return guard([&] {
+    const ki::Ctx& c = ctx->c;
+    const int n = c.labellog.size();
+    if (count) *count = n;
+    const int nc = std::max(0, std::min(cap, n));
+    if (feat && nc > 0)
+      std::memcpy(feat, c.featlog.data(), sizeof(float) * ki::MLP_NF * nc);
+    if (label && nc > 0)
+      std::memcpy(label, c.labellog.data(), sizeof(float) * nc);
+    if (pred && nc > 0) std::memcpy(pred, c.predlog.data(), sizeof(float) * nc);
+  });
+}
+
+static void mlp_alloc(ki::Ctx& c) {
+  const size_t b = sizeof(float) * ki::MLP_NPARAM;
+  c.mlp_p.ensure(b);
+  c.mlp_m.ensure(b);
+  c.mlp_v.ensure(b);
+  c.mlp_g.ensure(b + sizeof(float));
+}
+
+int ki_mlp_set_params(ki_ctx* ctx, const float* params, int count) {
+  return guard([&] {
+    KI_CHECK(params && count == ki::MLP_NPARAM, KI_E_ARG,
+             "expected KI_MLP_NPARAM parameters");
+    ki::Ctx& c = ctx->c;
+    mlp_alloc(c);
+    KI_HIP(hipMemcpy(c.mlp_p.p, params, sizeof(float) * count,
+                     hipMemcpyHostToDevice));
+    c.mlp_ready = true;
+  });
+}
+
+int ki_mlp_get_params(ki_ctx* ctx, float* params, int count) {
+  return guard([&] {
+    KI_CHECK(params && count == ki::MLP_NPARAM, KI_E_ARG,
+             "expected KI_MLP_NPARAM parameters");
+    KI_CHECK(ctx->c.mlp_ready, KI_E_STATE, "predictor has no parameters");
+    KI_HIP(hipMemcpy(params, ctx->c.mlp_p.p, sizeof(float) * count,
+                     hipMemcpyDeviceToHost));
+  });
+}
+
+int ki_mlp_init(ki_ctx* ctx, uint64_t seed) {
+  return guard([&] {
+    using namespace ki;
+    std::vector<float> p(MLP_NPARAM, 0.0f);
+    std::mt19937_64 rng(seed);
+    auto fill = [&](int off, int fan_out, int fan_in) {
+      const double a = std::sqrt(6.0 / (fan_in + fan_out));
+      std::uniform_real_distribution<double> u(-a, a);
+      for (int e = 0; e < fan_out * fan_in; e++) p[off + e] = (float)u(rng);
+    };
+    const int ow1 = 0, ow2 = MLP_NH * MLP_NF + MLP_NH,
+              ow3 = ow2 + MLP_NH * MLP_NH + MLP_NH;
+    fill(ow1, MLP_NH, MLP_NF);
+    fill(ow2, MLP_NH, MLP_NH);
+    fill(ow3, 1, MLP_NH);
+    Ctx& c = ctx->c;
+    mlp_alloc(c);
+    KI_HIP(hipMemcpy(c.mlp_p.p, p.data(), sizeof(float) * MLP_NPARAM,
+                     hipMemcpyHostToDevice));
+    c.mlp_ready = true;
+  });
+}
+
+int ki_mlp_grad(ki_ctx* ctx, const float* X, const float* y, int n,
+                float* grad, float* loss) {
+  return guard([&] {
+    using namespace ki;
+    Ctx& c = ctx->c;
+    KI_CHECK(c.mlp_ready, KI_E_STATE, "predictor has no parameters");
+    KI_CHECK(X && y && n > 0, KI_E_ARG, "empty training set");
+    std::vector<float> hx(X, X + (size_t)n * MLP_NF), hy(y, y + n);
+    upload(c.mlp_X, hx, c.s);
+    upload(c.mlp_y, hy, c.s);
+    c.mlp_part.ensure(sizeof(float) * (MLP_NPARAM + 1) * mlp_partials(n));
+    mlp_grad(c.mlp_p.as<float>(), c.mlp_X.as<float>(), c.mlp_y.as<float>(),
+             nullptr, n, c.mlp_part.as<float>(), c.mlp_g.as<float>(), c.s);
+    std::vector<float> g(MLP_NPARAM + 1);
+    KI_HIP(hipMemcpyAsync(g.data(), c.mlp_g.p, sizeof(float) * g.size(),
+                          hipMemcpyDeviceToHost, c.s));
+    KI_HIP(hipStreamSynchronize(c.s));
+    if (grad) std::memcpy(grad, g.data(), sizeof(float) * MLP_NPARAM);
+    if (loss) *loss = g[MLP_NPARAM] / n;
+  });
+}
+
+int ki_mlp_train(ki_ctx* ctx, const float* X, const float* y, int n,
+                 int epochs, int batch, float lr, uint64_t seed, float* loss) {
+  return guard([&] {
+    using namespace ki;
+    Ctx& c = ctx->c;
+    KI_CHECK(c.mlp_ready, KI_E_STATE, "predictor has no parameters");
+    KI_CHECK(X && y && n > 0 && epochs >= 0 && batch > 0 && lr > 0, KI_E_ARG,
+             "bad training arguments");
+    std::vector<float> hx(X, X + (size_t)n * MLP_NF), hy(y, y + n);
+    upload(c.mlp_X, hx, c.s);
+    upload(c.mlp_y, hy, c.s);
+    launch_zero<float>(c.mlp_m.as<float>(), MLP_NPARAM, c.s);
+    launch_zero<float>(c.mlp_v.as<float>(), MLP_NPARAM, c.s);
+    const int bs = std::min(batch, n);
+    c.mlp_part.ensure(sizeof(float) * (MLP_NPARAM + 1) *
+                      std::max(mlp_partials(bs), mlp_partials(n)));
+    std::mt19937_64 rng(seed);
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    int t = 0;
+    for (int e = 0; e < epochs; e++) {
+      std::shuffle(order.begin(), order.end(), rng);
+      upload(c.mlp_idx, order, c.s);
+      for (int b0 = 0; b0 < n; b0 += bs) {
+        const int nb = std::min(bs, n - b0);
+        mlp_grad(c.mlp_p.as<float>(), c.mlp_X.as<float>(),
+                 c.mlp_y.as<float>(), c.mlp_idx.as<int>() + b0, nb,
+                 c.mlp_part.as<float>(), c.mlp_g.as<float>(), c.s);
+        mlp_adam(c.mlp_p.as<float>(), c.mlp_g.as<float>(),
+                 c.mlp_m.as<float>(), c.mlp_v.as<float>(), lr, ++t, c.s);
+      }
+      // the index upload of the next epoch reuses the buffer
+      KI_HIP(hipStreamSynchronize(c.s));
+    }
+    mlp_grad(c.mlp_p.as<float>(), c.mlp_X.as<float>(), c.mlp_y.as<float>(),
+             nullptr, n, c.mlp_part.as<float>(), c.mlp_g.as<float>(), c.s);
+    float l = 0;
+    KI_HIP(hipMemcpyAsync(&l, c.mlp_g.as<float>() + MLP_NPARAM, sizeof(float),
+                          hipMemcpyDeviceToHost, c.s));
+    KI_HIP(hipStreamSynchronize(c.s));
+    if (loss) *loss = l / n;
+  });
+}
+
+int ki_mlp_predict(ki_ctx* ctx, const float* X, int n, int bf16, float* out) {
+  return guard([&] {
+    using namespace ki;
+    Ctx& c = ctx->c;
+    KI_CHECK(c.mlp_ready, KI_E_STATE, "predictor has no parameters");
+    KI_CHECK(X && out && n >= 0, KI_E_ARG, "bad arguments");
+    if (n == 0) return;
+    std::vector<float> hx(X, X + (size_t)n * MLP_NF);
+    upload(c.mlp_X, hx, c.s);
+    c.mlp_out.ensure(sizeof(float) * n);
+    mlp_infer(c.mlp_p.as<float>(), c.mlp_X.as<float>(), n,
+              c.mlp_out.as<float>(), bf16 ? 1 : 0, c.s);
+    KI_HIP(hipMemcpyAsync(out, c.mlp_out.p, sizeof(float) * n,
+                          hipMemcpyDeviceToHost, c.s));
+    KI_HIP(hipStreamSynchronize(c.s));
+  });
+}
+
+int ki_kernel_clock(ki_ctx* ctx, int i, char* name, int name_cap,
+                    double* vals, int* count) {
+  return guard([&] {
+    const auto& v = ctx->c.kclk;
+    if (count) *count = v.size();
+    if (i < 0) return;
+    KI_CHECK(i < (int)v.size(), KI_E_ARG, "kernel clock index out of range");
+    if (name && name_cap > 0) {
+      std::snprintf(name, name_cap, "%s", v[i].name.c_str());
+    }
+    if (vals) {
+      vals[0] = (double)v[i].launches;
+      vals[1] = v[i].seconds;
+      vals[2] = v[i].flops;
+      vals[3] = v[i].bytes;
+    }
   });
 }
 

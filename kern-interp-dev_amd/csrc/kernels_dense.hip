@@ -8,7 +8,10 @@
 // (ki_mat.cpp:328-335, 445-513) inside decouple (skel_factorization.cpp:
 // 91-127), the top-level LU (:285-310) and the block-diagonal solves of the
 // sweep (:585-592, 676-722).
+#include <hip/hip_cooperative_groups.h>
+
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdlib>
 
@@ -283,6 +286,140 @@ lu_pstep(T* __restrict__ A, int ld, int m, int j0, int jb,
   }
 }
 
+// The whole column-panel factorization A[j0:m, j0:j0+jb] of one tall matrix in
+// a single cooperative launch.  Workgroup b keeps rows [b*PR, b*PR + PR) of
+// the panel in LDS for all jb column steps.  Per column c: every workgroup
+// posts its best pivot candidate (|value|, row, and the row's panel entries)
+// and the owner of row c posts row c; one grid barrier; then every workgroup
+// picks the same pivot p (first row of maximal |a|, as idamax), the owners of
+// rows c and p exchange them, and each workgroup applies the rank-1 update to
+// its rows below c.  Posted data goes to per-step slots, so no slot is
+// rewritten while another workgroup may still read it.  Same interchanges as
+// the one-workgroup panel (LAPACK getf2 order).
+constexpr int PR = 224;  // panel rows per workgroup (LDS: PR x LB values)
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+lu_coop_panel(T* __restrict__ A, int ld, int m, int j0, int jb,
+              int* __restrict__ piv, T* __restrict__ cv, int* __restrict__ ci,
+              T* __restrict__ crow, T* __restrict__ rowc) {
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  const int nwg = gridDim.x, b = blockIdx.x, tid = threadIdx.x;
+  const int mr = m - j0;
+  const int r0 = b * PR;
+  const int nr = min(PR, mr - r0);
+  __shared__ T P[LB][PR + 1];  // P[q][i] = A[j0 + r0 + i, j0 + q]
+  __shared__ T prow[LB];
+  __shared__ T rv[4];
+  __shared__ int ri[4];
+  __shared__ int s_p;
+  for (int e = tid; e < LB * PR; e += 256) {
+    const int i = e % PR, q = e / PR;
+    P[q][i] = (i < nr && q < jb) ? A[(j0 + r0 + i) + (int64_t)ld * (j0 + q)]
+                                 : (T)0;
+  }
+  __syncthreads();
+  const int lane = tid & 63, wid = tid >> 6;
+  for (int c = 0; c < jb; c++) {
+    // local candidate: first local row of maximal |a| among rows >= c
+    T best = -1;
+    int bi = INT_MAX;
+    if (tid < nr && r0 + tid >= c) {
+      best = fabs(P[c][tid]);
+      bi = r0 + tid;
+      if (!(best == best)) best = -1;  // NaN never wins
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      const T ov = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      if (ov > best || (ov == best && oi < bi)) {
+        best = ov;
+        bi = oi;
+      }
+    }
+    if (lane == 0) {
+      rv[wid] = best;
+      ri[wid] = bi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      T bb = rv[0];
+      int ii = ri[0];
+      for (int w = 1; w < 4; w++)
+        if (rv[w] > bb || (rv[w] == bb && ri[w] < ii)) {
+          bb = rv[w];
+          ii = ri[w];
+        }
+      s_p = ii;
+      cv[(int64_t)c * nwg + b] = bb;
+      ci[(int64_t)c * nwg + b] = ii;
+    }
+    __syncthreads();
+    const int mine = s_p;
+    if (mine != INT_MAX && tid < jb)
+      crow[((int64_t)c * nwg + b) * LB + tid] = P[tid][mine - r0];
+    if (c >= r0 && c < r0 + nr && tid < jb)
+      rowc[(int64_t)c * LB + tid] = P[tid][c - r0];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    grid.sync();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    // global pivot: the same reduction order in every workgroup
+    if (tid < 64) {
+      T bb = -1;
+      int ii = INT_MAX, ww = -1;
+      for (int w = tid; w < nwg; w += 64) {
+        const T v = cv[(int64_t)c * nwg + w];
+        const int x = ci[(int64_t)c * nwg + w];
+        if (v > bb || (v == bb && x < ii)) {
+          bb = v;
+          ii = x;
+          ww = w;
+        }
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        const T ov = __shfl_xor(bb, o, 64);
+        const int oi = __shfl_xor(ii, o, 64);
+        const int ow = __shfl_xor(ww, o, 64);
+        if (ov > bb || (ov == bb && oi < ii)) {
+          bb = ov;
+          ii = oi;
+          ww = ow;
+        }
+      }
+      // no finite candidate (or all zero entries tie at row c): keep row c
+      int p = (ii == INT_MAX || ii < c || ii >= mr || !(bb > 0)) ? c : ii;
+      if (tid < jb)
+        prow[tid] = p == c ? rowc[(int64_t)c * LB + tid]
+                           : crow[((int64_t)c * nwg + ww) * LB + tid];
+      if (tid == 0) {
+        s_p = p;
+        if (b == 0) piv[j0 + c] = j0 + p;
+      }
+    }
+    __syncthreads();
+    const int p = s_p;
+    // interchange rows c and p
+    if (p != c && tid < jb) {
+      if (p >= r0 && p < r0 + nr)
+        P[tid][p - r0] = rowc[(int64_t)c * LB + tid];
+      if (c >= r0 && c < r0 + nr) P[tid][c - r0] = prow[tid];
+    }
+    __syncthreads();
+    // scale and rank-1 update of local rows below c
+    const T pv = prow[c];
+    if (tid < nr && r0 + tid > c) {
+      const T l = pv != (T)0 ? P[c][tid] / pv : (T)0;
+      P[c][tid] = l;
+      for (int q = c + 1; q < jb; q++) P[q][tid] -= l * prow[q];
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < LB * PR; e += 256) {
+    const int i = e % PR, q = e / PR;
+    if (i < nr && q < jb) A[(j0 + r0 + i) + (int64_t)ld * (j0 + q)] = P[q][i];
+  }
+}
+
 // Row interchanges piv[r0..r1) applied, in order, to columns [clo, chi) of a
 // single matrix (one thread per column).
 template <typename T>
@@ -372,18 +509,23 @@ lu_swap_trsm(const MatDesc* __restrict__ desc, T* __restrict__ a_all,
 // Triangular solves with the LU factors, blocked by LB rows:
 //   mode 0: L (unit lower) forward      mode 1: U (upper) backward
 //   mode 2: U^T (lower) forward         mode 3: L^T (unit upper) backward
-// The diagonal-block kernel stages the LB x LB triangle in LDS and solves it
-// for every right-hand-side column (one thread per column, the column's LB
-// entries in registers); GEMM updates the remaining rows.
+// The diagonal-block kernel stages the LB x LB triangle and a block of TC
+// right-hand-side columns in LDS (coalesced column reads), solves each column
+// with one thread, and writes the block back; GEMM updates the other rows.
 // ---------------------------------------------------------------------------
+constexpr int TC = 128;  // right-hand-side columns per workgroup
+
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(TC)
 trsm_diag(const MatDesc* __restrict__ desc, const T* __restrict__ a_all,
           T* __restrict__ b_all, int blk, int mode) {
   const MatDesc d = desc[blockIdx.y];
   const int n = d.n;
   const int nblk = (n + LB - 1) / LB;
   if (blk >= nblk || d.nrhs <= 0) return;
+  const int c0 = blockIdx.x * TC;
+  if (c0 >= d.nrhs) return;
+  const int nc = min(TC, d.nrhs - c0);
   // forward modes walk blocks top-down, backward modes bottom-up
   const int b = (mode == 0 || mode == 2) ? blk : nblk - 1 - blk;
   const int r0 = b * LB, rb = min(LB, n - r0);
@@ -391,39 +533,48 @@ trsm_diag(const MatDesc* __restrict__ desc, const T* __restrict__ a_all,
   const int ld = d.ld;
   // D[t][q] = A(r0 + t, r0 + q); transposed modes read D[q][t]
   __shared__ T D[LB][LB + 1];
-  for (int e = threadIdx.x; e < LB * LB; e += blockDim.x) {
+  __shared__ T X[LB][TC + 1];  // X[t][j] = B(r0 + t, c0 + j)
+  for (int e = threadIdx.x; e < LB * LB; e += TC) {
     const int t = e % LB, q = e / LB;  // identity outside the rb x rb block
     D[t][q] = (t < rb && q < rb) ? A[(r0 + t) + (int64_t)ld * (r0 + q)]
                                  : (T)(t == q ? 1 : 0);
   }
+  T* Bp = b_all + d.b_off + r0 + (int64_t)d.ldb * c0;
+  for (int e = threadIdx.x; e < LB * TC; e += TC) {
+    const int t = e % LB, j = e / LB;
+    X[t][j] = (t < rb && j < nc) ? Bp[t + (int64_t)d.ldb * j] : (T)0;
+  }
   __syncthreads();
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= d.nrhs) return;
-  // Plain loops over the column in place (a fully unrolled register-array
-  // form of the same solve was miscomputed on gfx950).
-  T* x = b_all + d.b_off + (int64_t)d.ldb * col + r0;
-  if (mode == 0) {  // unit lower
-    for (int q = 0; q < rb; q++) {
-      const T v = x[q];
-      for (int t = q + 1; t < rb; t++) x[t] -= D[t][q] * v;
+  const int j = threadIdx.x;
+  if (j < nc) {
+    if (mode == 0) {  // unit lower
+      for (int q = 0; q < rb; q++) {
+        const T v = X[q][j];
+        for (int t = q + 1; t < rb; t++) X[t][j] -= D[t][q] * v;
+      }
+    } else if (mode == 1) {  // upper, non-unit
+      for (int q = rb - 1; q >= 0; q--) {
+        const T v = X[q][j] / D[q][q];
+        X[q][j] = v;
+        for (int t = 0; t < q; t++) X[t][j] -= D[t][q] * v;
+      }
+    } else if (mode == 2) {  // U^T: lower, non-unit, (U^T)(t,q) = D[q][t]
+      for (int q = 0; q < rb; q++) {
+        const T v = X[q][j] / D[q][q];
+        X[q][j] = v;
+        for (int t = q + 1; t < rb; t++) X[t][j] -= D[q][t] * v;
+      }
+    } else {  // L^T: unit upper, (L^T)(t,q) = D[q][t]
+      for (int q = rb - 1; q >= 0; q--) {
+        const T v = X[q][j];
+        for (int t = 0; t < q; t++) X[t][j] -= D[q][t] * v;
+      }
     }
-  } else if (mode == 1) {  // upper, non-unit
-    for (int q = rb - 1; q >= 0; q--) {
-      const T v = x[q] / D[q][q];
-      x[q] = v;
-      for (int t = 0; t < q; t++) x[t] -= D[t][q] * v;
-    }
-  } else if (mode == 2) {  // U^T: lower, non-unit, (U^T)(t,q) = D[q][t]
-    for (int q = 0; q < rb; q++) {
-      const T v = x[q] / D[q][q];
-      x[q] = v;
-      for (int t = q + 1; t < rb; t++) x[t] -= D[q][t] * v;
-    }
-  } else {  // L^T: unit upper, (L^T)(t,q) = D[q][t]
-    for (int q = rb - 1; q >= 0; q--) {
-      const T v = x[q];
-      for (int t = 0; t < q; t++) x[t] -= D[q][t] * v;
-    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < LB * TC; e += TC) {
+    const int t = e % LB, jj = e / LB;
+    if (t < rb && jj < nc) Bp[t + (int64_t)d.ldb * jj] = X[t][jj];
   }
 }
 
@@ -562,6 +713,29 @@ static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
                           hipMemcpyHostToDevice, s));
   KI_HIP(hipMallocAsync((void**)&d_m, sizeof(MatDesc), s));
   KI_HIP(hipMallocAsync((void**)&W, sizeof(T) * (size_t)m * LB, s));
+  // Cooperative panels need every workgroup of the grid resident at once.
+  const int coop_wg = (m + PR - 1) / PR;
+  bool coop = std::getenv("KI_LU_NO_COOP") == nullptr;
+  if (coop) {
+    int dev = 0, ncu = 0, per_cu = 0, can = 0;
+    KI_HIP(hipGetDevice(&dev));
+    KI_HIP(hipDeviceGetAttribute(&can, hipDeviceAttributeCooperativeLaunch,
+                                 dev));
+    KI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                                 dev));
+    KI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(&lu_coop_panel<T>), 256, 0));
+    coop = can && (int64_t)per_cu * ncu >= coop_wg;
+  }
+  T *cv = nullptr, *crow = nullptr, *rowc = nullptr;
+  int* ci = nullptr;
+  if (coop) {
+    KI_HIP(hipMallocAsync((void**)&cv, sizeof(T) * (size_t)LB * coop_wg, s));
+    KI_HIP(hipMallocAsync((void**)&ci, sizeof(int) * (size_t)LB * coop_wg, s));
+    KI_HIP(hipMallocAsync((void**)&crow,
+                          sizeof(T) * (size_t)LB * coop_wg * LB, s));
+    KI_HIP(hipMallocAsync((void**)&rowc, sizeof(T) * (size_t)LB * LB, s));
+  }
   size_t gi = 0;
   auto cols = [](int c) { return dim3((std::max(c, 1) + 255) / 256); };
   for (int J0 = 0; J0 < mn; J0 += NB) {
@@ -570,12 +744,21 @@ static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
       const int jb = std::min(LB, J0 + JB - j0);
       const int mr = m - j0;
       const dim3 g((mr + 255) / 256);
-      for (int c = 0; c < jb; c++)
-        hipLaunchKernelGGL(lu_pstep<T>, g, dim3(256), 0, s, A, ld, m, j0, jb,
-                           W, c, piv, c & 1);
-      if (jb & 1)
-        hipLaunchKernelGGL(panel_copy_back<T>, g, dim3(256), 0, s, A, ld, m,
-                           j0, jb, (const T*)W);
+      if (coop) {
+        int a_ld = ld, a_m = m, a_j0 = j0, a_jb = jb;
+        void* args[] = {&A,  &a_ld, &a_m, &a_j0, &a_jb, &piv,
+                        &cv, &ci,   &crow, &rowc};
+        KI_HIP(hipLaunchCooperativeKernel(
+            reinterpret_cast<const void*>(&lu_coop_panel<T>),
+            dim3((mr + PR - 1) / PR), dim3(256), args, 0, s));
+      } else {
+        for (int c = 0; c < jb; c++)
+          hipLaunchKernelGGL(lu_pstep<T>, g, dim3(256), 0, s, A, ld, m, j0, jb,
+                             W, c, piv, c & 1);
+        if (jb & 1)
+          hipLaunchKernelGGL(panel_copy_back<T>, g, dim3(256), 0, s, A, ld, m,
+                             j0, jb, (const T*)W);
+      }
       if (j0 > J0)
         hipLaunchKernelGGL(swap_cols_range<T>, cols(j0 - J0), dim3(256), 0, s,
                            A, ld, (const int*)piv, j0, j0 + jb, J0, j0);
@@ -621,6 +804,12 @@ static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
   KI_HIP(hipFreeAsync(d_g, s));
   KI_HIP(hipFreeAsync(d_m, s));
   KI_HIP(hipFreeAsync(W, s));
+  if (coop) {
+    KI_HIP(hipFreeAsync(cv, s));
+    KI_HIP(hipFreeAsync(ci, s));
+    KI_HIP(hipFreeAsync(crow, s));
+    KI_HIP(hipFreeAsync(rowc, s));
+  }
 }
 
 template <typename T>
@@ -766,8 +955,8 @@ static void trsm_sweeps(const std::vector<MatDesc>& h, MatDesc* d_desc,
   for (int mi = 0; mi < nmodes; mi++) {
     for (int blk = 0; blk < nblk; blk++) {
       const int slot = mi * nblk + blk;
-      hipLaunchKernelGGL(trsm_diag<T>, cgrid, dim3(256), 0, s, d_desc, a, b,
-                         blk, modes[mi]);
+      hipLaunchKernelGGL(trsm_diag<T>, dim3((maxr + TC - 1) / TC, nb),
+                         dim3(TC), 0, s, d_desc, a, b, blk, modes[mi]);
       if (mm[slot] > 0 && nn[slot] > 0)
         launch_gemm<T>(d_g + (size_t)slot * nb, nb, mm[slot], nn[slot], a, b,
                        b, s);

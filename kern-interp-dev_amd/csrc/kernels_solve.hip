@@ -4,14 +4,17 @@
 //   up   (levels L-1..0):  x[r] -= T^T x[s];  x[s] -= L x[r]
 //   diag (all boxes):      x[r] = X_rr^{-1} x[r]
 //   down (levels 0..L-1):  x[r] -= U x[s];   x[s] -= T x[r]
-// One workgroup per compressed box; boxes of a level own disjoint DOFs, so a
-// level is one launch.  L is stored transposed (r x k) so that both sweeps
-// read every factor with coalesced loads.
+// Boxes of a level own disjoint DOFs, so each sweep phase of a level is one
+// launch, split into work items of a few dozen outputs per workgroup.  L is
+// stored transposed (r x k) so that every phase reads its factor with
+// coalesced loads.  The diagonal and top-level LU solves are blocked
+// (256-row diagonal blocks, multi-workgroup updates of the other rows).
 //
 // The matvec is the FP64 residual operator K mu for the whole boundary (the
 // reference forms it densely in solve_err, linear_solve.cpp:339-426; here it
 // is an N-body loop over LDS-staged source tiles), and diag3d restates the
 // row-sum diagonals (kernel.cpp:124-301).
+#include <algorithm>
 #include <cmath>
 
 #include "ki_internal.h"
@@ -26,105 +29,183 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
+// ---------------------------------------------------------------------------
+// Multi-workgroup sweeps.  Each phase is a GEMV per box,
+//   phase 0 (up):   x[r] -= T^T x[s]   column form, M = T   (k x r, ld k)
+//   phase 1 (up):   x[s] -= L x[r]     column form, M = L^T (r x k, ld r)
+//   phase 2 (down): x[r] -= U x[s]     row form,    M = U   (r x k, ld r)
+//   phase 3 (down): x[s] -= T x[r]     row form,    M = T   (k x r, ld k)
+// split into work items of SW_COL / SW_ROW outputs so that a level with a
+// few large boxes still fills the chip.  Every workgroup gathers its box's
+// input vector into LDS; outputs of one phase are disjoint from its inputs.
+// ---------------------------------------------------------------------------
+constexpr int SW_COL = 32, SW_ROW = 64;
+
 template <typename T>
-__global__ void __launch_bounds__(256)
-sweep_kernel(const SweepDesc* __restrict__ desc, const int* __restrict__ idx,
-             T* const* __restrict__ facbufs, T* __restrict__ x, int up) {
-  const SweepDesc d = desc[blockIdx.x];
-  const int k = d.k, r = d.r;
-  extern __shared__ char smem[];
-  T* xs = reinterpret_cast<T*>(smem);
-  T* xr = xs + k;
+__device__ __forceinline__ void sweep_operands(const SweepDesc& d, int phase,
+                                               const int* idx,
+                                               T* const* facbufs, const T*& M,
+                                               int& len, int& nout,
+                                               const int*& in,
+                                               const int*& out) {
+  const T* F = facbufs[d.buf];
   const int* si = idx + d.s_idx;
   const int* ri = idx + d.r_idx;
-  const T* F = facbufs[d.buf];
-  const T* Tm = F + d.offT;   // k x r
-  const T* LT = F + d.offL;   // r x k (L transposed)
-  const T* U = F + d.offU;    // r x k
-  for (int i = threadIdx.x; i < k; i += 256) xs[i] = x[si[i]];
-  for (int j = threadIdx.x; j < r; j += 256) xr[j] = x[ri[j]];
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (up) {
-    // x[r] -= T^T x[s]: column j of T dotted with xs (wave per column)
-    for (int j = wid; j < r; j += 4) {
-      const T* c = Tm + (int64_t)k * j;
-      T p = 0;
-      for (int i = lane; i < k; i += 64) p += c[i] * xs[i];
-      p = wave_sum(p);
-      if (lane == 0) xr[j] -= p;
-    }
-    __syncthreads();
-    // x[s] -= L x[r]: (L x_r)_i = column i of L^T dotted with xr
-    for (int i = wid; i < k; i += 4) {
-      const T* c = LT + (int64_t)r * i;
-      T p = 0;
-      for (int j = lane; j < r; j += 64) p += c[j] * xr[j];
-      p = wave_sum(p);
-      if (lane == 0) xs[i] -= p;
-    }
-  } else {
-    // x[r] -= U x[s]: thread per row j, coalesced over j
-    for (int j = threadIdx.x; j < r; j += 256) {
-      T p = 0;
-      for (int i = 0; i < k; i++) p += U[j + (int64_t)r * i] * xs[i];
-      xr[j] -= p;
-    }
-    __syncthreads();
-    // x[s] -= T x[r]: thread per row i of T
-    for (int i = threadIdx.x; i < k; i += 256) {
-      T p = 0;
-      for (int j = 0; j < r; j++) p += Tm[i + (int64_t)k * j] * xr[j];
-      xs[i] -= p;
-    }
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < k; i += 256) x[si[i]] = xs[i];
-  for (int j = threadIdx.x; j < r; j += 256) x[ri[j]] = xr[j];
+  const bool to_r = phase == 0 || phase == 2;
+  in = to_r ? si : ri;
+  out = to_r ? ri : si;
+  len = to_r ? d.k : d.r;
+  nout = to_r ? d.r : d.k;
+  M = F + (phase == 0 || phase == 3 ? d.offT : phase == 1 ? d.offL : d.offU);
 }
 
-// x[r] = X^{-1} x[r] with X = P L U (LAPACK getrf layout), one box per WG.
 template <typename T>
 __global__ void __launch_bounds__(256)
-diag_solve_kernel(const SweepDesc* __restrict__ desc,
-                  const int* __restrict__ idx, T* const* __restrict__ facbufs,
-                  int* const* __restrict__ pivbufs, T* __restrict__ x) {
-  const SweepDesc d = desc[blockIdx.x];
-  const int r = d.r;
-  if (r == 0) return;
+sweep_col_kernel(const SweepDesc* __restrict__ desc,
+                 const SweepItem* __restrict__ items,
+                 const int* __restrict__ idx, T* const* __restrict__ facbufs,
+                 T* __restrict__ x, int phase) {
+  const SweepItem it = items[blockIdx.x];
+  const SweepDesc d = desc[it.d];
+  const T* M;
+  int len, nout;
+  const int *in, *out;
+  sweep_operands<T>(d, phase, idx, facbufs, M, len, nout, in, out);
   extern __shared__ char smem[];
-  T* xr = reinterpret_cast<T*>(smem);
-  const int* ri = idx + d.r_idx;
-  const T* X = facbufs[d.buf] + d.offX;
-  const int* piv = pivbufs[d.buf] + d.offPiv;
-  for (int j = threadIdx.x; j < r; j += 256) xr[j] = x[ri[j]];
+  T* xin = reinterpret_cast<T*>(smem);
+  for (int i = threadIdx.x; i < len; i += 256) xin[i] = x[in[i]];
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int q = 0; q < r; q++) {
-      const int p = piv[q];
-      if (p != q) {
-        const T t = xr[q];
-        xr[q] = xr[p];
-        xr[p] = t;
-      }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int o1 = min(nout, it.o0 + SW_COL);
+  for (int j = it.o0 + wid; j < o1; j += 4) {
+    const T* c = M + (int64_t)len * j;
+    T p = 0;
+    for (int i = lane; i < len; i += 64) p += c[i] * xin[i];
+    p = wave_sum(p);
+    if (lane == 0) x[out[j]] -= p;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+sweep_row_kernel(const SweepDesc* __restrict__ desc,
+                 const SweepItem* __restrict__ items,
+                 const int* __restrict__ idx, T* const* __restrict__ facbufs,
+                 T* __restrict__ x, int phase) {
+  const SweepItem it = items[blockIdx.x];
+  const SweepDesc d = desc[it.d];
+  const T* M;
+  int len, nout;
+  const int *in, *out;
+  sweep_operands<T>(d, phase, idx, facbufs, M, len, nout, in, out);
+  extern __shared__ char smem[];
+  T* xin = reinterpret_cast<T*>(smem);
+  __shared__ T part[4][SW_ROW];
+  for (int i = threadIdx.x; i < len; i += 256) xin[i] = x[in[i]];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int row = it.o0 + lane;
+  const int q = (len + 3) / 4;
+  const int j0 = wid * q, j1 = min(len, j0 + q);
+  T p = 0;
+  if (row < nout)
+    for (int j = j0; j < j1; j++) p += M[row + (int64_t)nout * j] * xin[j];
+  part[wid][lane] = p;
+  __syncthreads();
+  if (wid == 0 && row < nout)
+    x[out[row]] -= part[0][lane] + part[1][lane] + part[2][lane] +
+                   part[3][lane];
+}
+
+// ---------------------------------------------------------------------------
+// Blocked LU solves w = X^{-1} w for many boxes at once (LAPACK getrs order:
+// the row interchanges are folded into the gather permutation).  Blocks of
+// DB rows: one wave stages the diagonal block in LDS (coalesced) and solves
+// it, then many workgroups apply the block's columns to the rows still to
+// be solved.
+// ---------------------------------------------------------------------------
+constexpr int DB = 64;
+
+template <typename TX, typename TW>
+__global__ void diag_gather_kernel(const TX* __restrict__ x,
+                                   const int* __restrict__ gidx, int64_t n,
+                                   TW* __restrict__ w) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x)
+    w[e] = (TW)x[gidx[e]];
+}
+
+template <typename TW, typename TX>
+__global__ void diag_scatter_kernel(const TW* __restrict__ w,
+                                    const int* __restrict__ sidx, int64_t n,
+                                    TX* __restrict__ x) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x)
+    x[sidx[e]] = (TX)w[e];
+}
+
+// Diagonal block b (forward: b-th from the top of the unit-lower L; backward:
+// b-th from the bottom of the upper U), one 64-lane workgroup per box.
+template <typename T>
+__global__ void __launch_bounds__(64)
+trsv_diag_kernel(const DiagDesc* __restrict__ desc, T* const* __restrict__ fac,
+                 T* __restrict__ w_all, int b, int upper) {
+  const DiagDesc d = desc[blockIdx.x];
+  const int r = d.r, nb = (r + DB - 1) / DB;
+  if (b >= nb) return;
+  const int bb = upper ? nb - 1 - b : b;
+  const int r0 = bb * DB, rb = min(DB, r - r0);
+  const T* X = fac[d.buf] + d.offX;
+  T* w = w_all + d.w_off;
+  __shared__ T ws[DB];
+  __shared__ T D[DB][DB + 1];  // D[t][q] = X(r0 + t, r0 + q)
+  const int t = threadIdx.x;
+  for (int q = 0; q < rb; q++)
+    if (t < rb) D[t][q] = X[(r0 + t) + (int64_t)r * (r0 + q)];
+  if (t < rb) ws[t] = w[r0 + t];
+  __syncthreads();
+  if (!upper) {
+    for (int q = 0; q < rb; q++) {
+      const T v = ws[q];
+      if (t > q && t < rb) ws[t] -= D[t][q] * v;
+      __syncthreads();
+    }
+  } else {
+    for (int q = rb - 1; q >= 0; q--) {
+      const T v = ws[q] / D[q][q];
+      __syncthreads();
+      if (t == q) ws[q] = v;
+      if (t < q) ws[t] -= D[t][q] * v;
+      __syncthreads();
     }
   }
+  if (t < rb) w[r0 + t] = ws[t];
+}
+
+// Rows outside diagonal block b: forward subtracts L(rows below, block) w_b,
+// backward subtracts U(rows above, block) w_b.  Grid (row chunks, boxes).
+template <typename T>
+__global__ void __launch_bounds__(256)
+trsv_update_kernel(const DiagDesc* __restrict__ desc,
+                   T* const* __restrict__ fac, T* __restrict__ w_all, int b,
+                   int upper) {
+  const DiagDesc d = desc[blockIdx.y];
+  const int r = d.r, nb = (r + DB - 1) / DB;
+  if (b >= nb) return;
+  const int bb = upper ? nb - 1 - b : b;
+  const int r0 = bb * DB, rb = min(DB, r - r0);
+  const int lo = upper ? 0 : r0 + rb, hi = upper ? r0 : r;
+  const int row = lo + blockIdx.x * 256 + threadIdx.x;
+  if (lo + (int)blockIdx.x * 256 >= hi) return;
+  const T* X = fac[d.buf] + d.offX;
+  T* w = w_all + d.w_off;
+  __shared__ T ws[DB];
+  for (int t = threadIdx.x; t < rb; t += 256) ws[t] = w[r0 + t];
   __syncthreads();
-  for (int q = 0; q < r; q++) {  // unit lower, column-oriented
-    const T v = xr[q];
-    const T* c = X + (int64_t)r * q;
-    for (int t = q + 1 + threadIdx.x; t < r; t += 256) xr[t] -= c[t] * v;
-    __syncthreads();
-  }
-  for (int q = r - 1; q >= 0; q--) {  // upper
-    const T* c = X + (int64_t)r * q;
-    const T v = xr[q] / c[q];
-    __syncthreads();
-    if (threadIdx.x == 0) xr[q] = v;
-    for (int t = threadIdx.x; t < q; t += 256) xr[t] -= c[t] * v;
-    __syncthreads();
-  }
-  for (int j = threadIdx.x; j < r; j += 256) x[ri[j]] = xr[j];
+  if (row >= hi) return;
+  T p = 0;
+  for (int q = 0; q < rb; q++) p += X[row + (int64_t)r * (r0 + q)] * ws[q];
+  w[row] -= p;
 }
 
 // ---------------------------------------------------------------------------
@@ -311,25 +392,65 @@ __global__ void scatter_vec_kernel(const TI* in, const int* idx, int n,
 
 }  // namespace
 
+int sweep_items_per_item(int phase) { return phase < 2 ? SW_COL : SW_ROW; }
+
 template <typename T>
-void launch_sweep(const SweepDesc* d, int nd, const int* idx,
-                  T* const* facbufs, T* x, int up, hipStream_t s,
-                  int max_n) {
-  if (nd == 0) return;
-  hipLaunchKernelGGL(sweep_kernel<T>, dim3(nd), dim3(256),
-                     sizeof(T) * max_n, s, d, idx, facbufs, x, up);
+void launch_sweep_phase(const SweepDesc* d, const SweepItem* items, int n,
+                        const int* idx, T* const* facbufs, T* x, int phase,
+                        int max_len, hipStream_t s) {
+  if (n == 0) return;
+  const size_t lds = sizeof(T) * (size_t)std::max(max_len, 1);
+  if (phase < 2)
+    hipLaunchKernelGGL(sweep_col_kernel<T>, dim3(n), dim3(256), lds, s, d,
+                       items, idx, facbufs, x, phase);
+  else
+    hipLaunchKernelGGL(sweep_row_kernel<T>, dim3(n), dim3(256), lds, s, d,
+                       items, idx, facbufs, x, phase);
   KI_HIP(hipGetLastError());
 }
 
-template <typename T>
-void launch_diag_solve(const SweepDesc* d, int nd, const int* idx,
-                       T* const* facbufs, int* const* pivbufs, T* x,
-                       hipStream_t s, int max_r) {
-  if (nd == 0) return;
-  hipLaunchKernelGGL(diag_solve_kernel<T>, dim3(nd), dim3(256),
-                     sizeof(T) * max_r, s, d, idx, facbufs, pivbufs, x);
+template <typename TX, typename TW>
+void launch_diag_blocked(const DiagDesc* d, int nd, int max_r,
+                         const int* gidx, const int* sidx, int64_t total,
+                         TW* const* facbufs, TX* x, TW* w, hipStream_t s) {
+  if (nd == 0 || total == 0) return;
+  const int gx = (int)std::min<int64_t>((total + 255) / 256, 8192);
+  hipLaunchKernelGGL((diag_gather_kernel<TX, TW>), dim3(gx), dim3(256), 0, s,
+                     (const TX*)x, gidx, total, w);
+  const int nb = (max_r + DB - 1) / DB;
+  const int chunks = (max_r + 255) / 256;
+  for (int upper = 0; upper < 2; upper++)
+    for (int b = 0; b < nb; b++) {
+      hipLaunchKernelGGL(trsv_diag_kernel<TW>, dim3(nd), dim3(64), 0, s, d,
+                         facbufs, w, b, upper);
+      if (b + 1 < nb)
+        hipLaunchKernelGGL(trsv_update_kernel<TW>, dim3(chunks, nd),
+                           dim3(256), 0, s, d, facbufs, w, b, upper);
+    }
+  hipLaunchKernelGGL((diag_scatter_kernel<TW, TX>), dim3(gx), dim3(256), 0, s,
+                     (const TW*)w, sidx, total, x);
   KI_HIP(hipGetLastError());
 }
+
+template void launch_sweep_phase<float>(const SweepDesc*, const SweepItem*,
+                                        int, const int*, float* const*,
+                                        float*, int, int, hipStream_t);
+template void launch_sweep_phase<double>(const SweepDesc*, const SweepItem*,
+                                         int, const int*, double* const*,
+                                         double*, int, int, hipStream_t);
+template void launch_diag_blocked<float, float>(const DiagDesc*, int, int,
+                                                const int*, const int*,
+                                                int64_t, float* const*,
+                                                float*, float*, hipStream_t);
+template void launch_diag_blocked<double, double>(const DiagDesc*, int, int,
+                                                  const int*, const int*,
+                                                  int64_t, double* const*,
+                                                  double*, double*,
+                                                  hipStream_t);
+template void launch_diag_blocked<float, double>(const DiagDesc*, int, int,
+                                                 const int*, const int*,
+                                                 int64_t, double* const*,
+                                                 float*, double*, hipStream_t);
 
 void launch_matvec(const DevGeom& g, const double* mu, double* y,
                    hipStream_t s) {
@@ -388,18 +509,6 @@ template void launch_scatter_vec<double, float>(const double*, const int*,
 template void launch_scatter_vec<double, double>(const double*, const int*,
                                                  int, double*, hipStream_t);
 
-template void launch_sweep<float>(const SweepDesc*, int, const int*,
-                                  float* const*, float*, int, hipStream_t,
-                                  int);
-template void launch_sweep<double>(const SweepDesc*, int, const int*,
-                                   double* const*, double*, int, hipStream_t,
-                                   int);
-template void launch_diag_solve<float>(const SweepDesc*, int, const int*,
-                                       float* const*, int* const*, float*,
-                                       hipStream_t, int);
-template void launch_diag_solve<double>(const SweepDesc*, int, const int*,
-                                        double* const*, int* const*, double*,
-                                        hipStream_t, int);
 template void launch_convert<float, double>(const double*, float*, int64_t,
                                             hipStream_t);
 template void launch_convert<double, float>(const float*, double*, int64_t,

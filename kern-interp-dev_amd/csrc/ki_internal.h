@@ -262,14 +262,48 @@ struct SweepDesc {
   int64_t offX, offPiv;
 };
 
+// One work item of a multi-workgroup sweep phase: descriptor d, outputs from
+// o0 (32 outputs per item in column-form phases 0-1, 64 in row-form 2-3).
+struct SweepItem {
+  int d, o0;
+};
+int sweep_items_per_item(int phase);
 template <typename T>
-void launch_sweep(const SweepDesc* d, int nd, const int* idx,
-                  T* const* facbufs, T* x, int up, hipStream_t s, int max_n);
+void launch_sweep_phase(const SweepDesc* d, const SweepItem* items, int n,
+                        const int* idx, T* const* facbufs, T* x, int phase,
+                        int max_len, hipStream_t s);
 
-template <typename T>
-void launch_diag_solve(const SweepDesc* d, int nd, const int* idx,
-                       T* const* facbufs, int* const* pivbufs, T* x,
-                       hipStream_t s, int max_r);
+// Blocked batched LU solve of r x r factors (getrf layout, factor buffer
+// `buf`, element offset offX) on a work vector segment at w_off.
+struct DiagDesc {
+  int r, buf;
+  int64_t offX, w_off;
+};
+// x[sidx[e]] = (X^{-1} gathered)[e] with w[e] = x[gidx[e]] first; gidx folds
+// the LU row interchanges.  TX: vector type, TW: factor / work type.
+template <typename TX, typename TW>
+void launch_diag_blocked(const DiagDesc* d, int nd, int max_r,
+                         const int* gidx, const int* sidx, int64_t total,
+                         TW* const* facbufs, TX* x, TW* w, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Learned rank predictor (12-64-64-1 perceptron, flat FP32 parameters).
+// ---------------------------------------------------------------------------
+constexpr int MLP_NF = 12, MLP_NH = 64;
+constexpr int MLP_NPARAM =
+    MLP_NH * MLP_NF + MLP_NH + MLP_NH * MLP_NH + MLP_NH + MLP_NH + 1;
+// Per-workgroup partial gradients needed for n samples.
+int mlp_partials(int n);
+// grad[0..NPARAM) = d(mean squared error)/d(params) over samples idx[0..n)
+// (or 0..n when idx is null); grad[NPARAM] = sum of squared errors.  part
+// holds mlp_partials(n) * (NPARAM + 1) floats.
+void mlp_grad(const float* params, const float* X, const float* y,
+              const int* idx, int n, float* part, float* grad, hipStream_t s);
+// Adam step t (1-based) with lr, beta1 0.9, beta2 0.999, eps 1e-8.
+void mlp_adam(float* params, const float* grad, float* m, float* v, float lr,
+              int t, hipStream_t s);
+void mlp_infer(const float* params, const float* X, int n, float* out,
+               int bf16, hipStream_t s);
 
 void launch_matvec(const DevGeom& g, const double* mu, double* y,
                    hipStream_t s);

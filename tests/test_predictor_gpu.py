@@ -1,0 +1,212 @@
+"""GPU checks of the learned rank predictor through the C-ABI.
+
+Bars: the GPU gradient and FP32 inference agree with the oracle
+(oracle/mlp_oracle.py, itself pinned by finite differences) to FP32
+round-off (relative 1e-4 / absolute 1e-5); BF16 inference within 3e-2
+absolute (BF16 keeps 8 significant bits); the harvested features equal the
+oracle's restatement from the box lists; a predictor that under-predicts
+every rank falls back to the full factorization and reproduces the plain
+skeletons exactly; one that predicts every box incompressible still solves
+to 1e-10; a trained predictor's gain over the plain form is measured.
+"""
+import time
+
+import numpy as np
+import pytest
+
+import mlp_oracle as M
+from conftest import geometry, pkg
+
+pytestmark = pytest.mark.gpu
+
+ki = pkg()
+G = geometry()
+
+
+def _rand(n, seed):
+    rng = np.random.default_rng(seed)
+    return (rng.uniform(0, 1, (n, M.NF)).astype(np.float32),
+            rng.uniform(0, 1, n).astype(np.float32))
+
+
+def _params(seed):
+    p = M.init_params(seed)
+    p[M.OB1:M.OW2] = np.random.default_rng(seed + 7).normal(0, 0.3, M.NH)
+    p[M.OB3] = 0.3
+    return p.astype(np.float32)
+
+
+def test_gradient_matches_oracle():
+    ctx = ki.Context()
+    X, y = _rand(1000, 1)          # 1000 samples: 32 partial sums + reduce
+    p = _params(2)
+    ctx.mlp_set_params(p)
+    g, loss = ctx.mlp_grad(X, y)
+    lo, go = M.loss_and_grad(p.astype(np.float64), X.astype(np.float64),
+                             y.astype(np.float64))
+    assert abs(loss - lo) <= 1e-5 * lo
+    assert np.linalg.norm(g - go) <= 1e-4 * np.linalg.norm(go)
+    for a, b in [(M.OW1, M.OB1), (M.OW2, M.OB2), (M.OW3, M.OB3)]:
+        assert np.linalg.norm(g[a:b] - go[a:b]) <= \
+            1e-4 * np.linalg.norm(go[a:b])
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 1000])
+def test_inference_fp32_and_bf16(n):
+    ctx = ki.Context()
+    X, _ = _rand(n, 3)
+    p = _params(4)
+    ctx.mlp_set_params(p)
+    ref = M.forward(p.astype(np.float64), X.astype(np.float64))
+    f32 = ctx.mlp_predict(X, bf16=False)
+    b16 = ctx.mlp_predict(X, bf16=True)
+    assert np.max(np.abs(f32 - ref)) <= 1e-5
+    assert np.max(np.abs(b16 - ref)) <= 3e-2
+
+
+def test_training_on_gpu_fits_a_smooth_target():
+    ctx = ki.Context()
+    X, _ = _rand(2048, 5)
+    y = (0.2 + 0.6 * X[:, 1] * (1 - X[:, 4]) + 0.1 * X[:, 7]).astype(
+        np.float32)
+    ctx.mlp_init(11)
+    l0 = float(np.mean((ctx.mlp_predict(X, bf16=False) - y) ** 2))
+    loss = ctx.mlp_train(X, y, epochs=60, batch=128, lr=3e-3, seed=1)
+    assert loss < 0.1 * l0 and loss < 2e-3
+    # the trained parameters mean the same function to the oracle
+    p = ctx.mlp_params().astype(np.float64)
+    assert np.max(np.abs(M.forward(p, X.astype(np.float64)) -
+                         ctx.mlp_predict(X, bf16=False))) <= 1e-5
+
+
+def _factor(b, pde, kw, leaf, predictor=0, params=None):
+    ctx = ki.Context()
+    ctx.set_geometry(b, pde)
+    ctx.build_tree(leaf)
+    if params is not None:
+        ctx.mlp_set_params(params)
+    prm = ki.default_params(use_predictor=predictor, **kw)
+    ctx.skeletonize(prm)
+    return ctx
+
+
+def _skeletons(ctx):
+    out = {}
+    for lv in range(ctx.num_levels()):
+        for i in range(ctx.level_size(lv)):
+            if ctx.box(lv, i)["compressed"]:
+                out[(lv, i)] = tuple(ctx.box_list(lv, i, "skel"))
+    return out
+
+
+def test_features_match_oracle_restatement():
+    b = G.star(32, 16)
+    kw = dict(id_tol=1e-8, proxy_points=128, proxy_ratio=1.5)
+    ctx = _factor(b, ki.KI_STOKES, kw, 128)
+    feat, lab, pred = ctx.box_features()
+    assert len(feat) > 0 and np.all(pred == -1)
+    # rebuild the feature rows from the box lists, in factorization order
+    rows = []
+    for lv in range(ctx.num_levels() - 1, 0, -1):
+        for i in range(ctx.level_size(lv)):
+            bx = ctx.box(lv, i)
+            act = ctx.box_list(lv, i, "active")
+            if len(act) < 64:
+                continue
+            rows.append((lv, i, bx, act))
+    assert len(rows) == len(feat)
+    for (lv, i, bx, act), f, y in zip(rows, feat, lab):
+        orig = ctx.box_list(lv, i, "original")
+        m = int(round(2 ** (f[3] * 16)))
+        special = lv <= 2
+        want = M.box_features(lv, act, orig, bx["side"], m, special, 1e-8, 2,
+                              2, b.pts(), b.nrm())
+        assert np.allclose(f, want, atol=2e-6), (lv, i)
+        k = bx["k"] if bx["compressed"] else len(act)
+        assert abs(y - k / len(act)) < 1e-6
+
+
+def test_underpredicting_falls_back_to_plain_skeletons():
+    b = G.star(64, 16)
+    kw = dict(id_tol=1e-8, proxy_points=128, proxy_ratio=1.5)
+    plain = _factor(b, ki.KI_STOKES, kw, 128)
+    p = np.zeros(M.NP, dtype=np.float32)
+    p[M.OB3] = -20.0                      # predicts k / n ~ 2e-9
+    low = _factor(b, ki.KI_STOKES, kw, 128, predictor=1, params=p)
+    st = low.stats()
+    assert st["boxes_predicted"] > 0 and st["boxes_fallback"] > 0
+    assert st["boxes_skipped"] == 0
+    assert _skeletons(low) == _skeletons(plain)
+    rhs = G.make_problem(b, "stokes", 1e-8, 128, 128, 1.5, 4, 1).rhs
+    x0, x1 = plain.solve(rhs), low.solve(rhs)
+    assert np.linalg.norm(x1 - x0) <= 1e-12 * np.linalg.norm(x0)
+
+
+def test_all_skipped_still_solves():
+    b = G.circle(1200)
+    kw = dict(id_tol=1e-8, proxy_points=128, proxy_ratio=2.0)
+    p = np.zeros(M.NP, dtype=np.float32)
+    p[M.OB3] = 20.0                       # predicts every box incompressible
+    ctx = _factor(b, ki.KI_LAPLACE, kw, 64, predictor=1, params=p)
+    st = ctx.stats()
+    assert st["boxes_compressed"] == 0 and st["boxes_skipped"] > 0
+    assert st["top_size"] == b.npts
+    f = G.make_problem(b, "laplace", 1e-8, 64, 128, 2.0, 5, 0).rhs
+    x, it, rel = ctx.gmres(f, tol=1e-10)
+    assert rel <= 1e-10 and it <= 2
+
+
+def _harvest(seeds, npts, kw, leaf):
+    Xs, ys = [], []
+    for sd in seeds:
+        b = G.random_smooth_curve(sd, npts(sd))
+        ctx = _factor(b, ki.KI_LAPLACE, kw, leaf)
+        f, y, _ = ctx.box_features()
+        Xs.append(f)
+        ys.append(y)
+    return np.concatenate(Xs), np.concatenate(ys)
+
+
+def test_trained_predictor_gain_on_smooth_curves():
+    """Train on held-out seeded curves, use on others: rank error, fallback
+    rate, skipped IDs and the factorization time against the plain form."""
+    kw = dict(id_tol=1e-8, proxy_points=128, proxy_ratio=1.5)
+    leaf = 64
+    rng = np.random.default_rng(0)
+    sizes = {sd: int(rng.integers(128, 513)) * 16 for sd in range(2000, 2024)}
+    X, y = _harvest(list(sizes), lambda sd: sizes[sd], kw, leaf)
+    ctx_t = ki.Context()
+    ctx_t.mlp_init(3)
+    loss = ctx_t.mlp_train(X, y, epochs=150, batch=256, lr=2e-3, seed=2)
+    params = ctx_t.mlp_params()
+    errs, fb, nb, skipped, t_plain, t_pred = [], 0, 0, 0, 0.0, 0.0
+    for sd in range(4):
+        b = G.random_smooth_curve(sd, 8192)
+        f = G.make_problem(b, "laplace", 1e-8, leaf, 128, 1.5, 5, 3).rhs
+        plain = _factor(b, ki.KI_LAPLACE, kw, leaf)
+        pr = _factor(b, ki.KI_LAPLACE, kw, leaf, predictor=1, params=params)
+        for c in (plain, pr):       # warm, then time one more factorization
+            t0 = time.perf_counter()
+            c.skeletonize(ki.default_params(
+                use_predictor=1 if c is pr else 0, **kw))
+            dt = time.perf_counter() - t0
+            if c is pr:
+                t_pred += dt
+            else:
+                t_plain += dt
+        fe, lab, pred = pr.box_features()
+        ok = lab >= 0
+        errs.append(np.abs(pred[ok] - lab[ok]))
+        st = pr.stats()
+        fb += st["boxes_fallback"]
+        nb += st["boxes_predicted"]
+        skipped += st["boxes_skipped"]
+        x, it, rel = pr.gmres(f, tol=1e-10)
+        assert rel <= 1e-10
+    mae = float(np.mean(np.concatenate(errs)))
+    print("[predictor] train loss %.2e on %d boxes; test MAE %.3f, fallback "
+          "%d / %d, skipped %d, factor %.3f s plain vs %.3f s predicted" %
+          (loss, len(y), mae, fb, nb, skipped, t_plain, t_pred), flush=True)
+    assert loss < 5e-3
+    assert mae < 0.08
+    assert fb <= 0.25 * nb
