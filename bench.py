@@ -70,6 +70,31 @@ def roofline(clock):
             "algorithmic_per_launch": work / k["launches"]}
 
 
+def train_predictor(ki, G, ctx, index, scale):
+    """Harvest (features, rank fraction) pairs from FP32 factorizations of
+    held-out smaller members of the workload's family (1/8, 1/4 and 1/2 of
+    its size), train the rank predictor on the GPU, load it into ctx."""
+    Xs, ys = [], []
+    t0 = time.perf_counter()
+    for s in (scale / 8, scale / 4, scale / 2):
+        hp = G.config(index, s)
+        h = ki.Context(0)
+        h.set_geometry(hp.boundary, ki.KI_LAPLACE if hp.pde == "laplace"
+                       else ki.KI_STOKES)
+        h.build_tree(hp.max_leaf_dofs)
+        h.skeletonize(params_for(ki, hp, 32))
+        f, y, _ = h.box_features()
+        Xs.append(f)
+        ys.append(y)
+        h.close()
+    X, y = np.concatenate(Xs), np.concatenate(ys)
+    ctx.mlp_init(7)
+    loss = ctx.mlp_train(X, y, epochs=300, batch=128, lr=2e-3, seed=7)
+    return {"train_boxes": int(len(y)), "train_loss": float(loss),
+            "train_s": time.perf_counter() - t0,
+            "held_out": "same family at 1/8, 1/4, 1/2 of the points"}
+
+
 def one_step(ctx, prm, tol, maxit, restart):
     t0 = time.perf_counter()
     ctx.skeletonize(prm)
@@ -122,6 +147,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--no-predictor", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the solver's multi-GPU "
@@ -136,13 +162,19 @@ def main():
     print("[bench] %s: %d points, %d DOFs" % (pb.name, b.npts, pb.ndof),
           file=sys.stderr, flush=True)
     ctx = ki.Context(0)
+    # learned ranks: trained before the timed region on held-out, smaller
+    # members of the same workload family
+    learned = None
+    if not args.no_predictor:
+        learned = train_predictor(ki, G, ctx, args.config, args.scale)
+        print("[bench] predictor: %s" % learned, file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     ctx.set_geometry(b, pde)      # uploads + 3D row-sum diagonals
     t_setup = time.perf_counter() - t0
     ctx.build_tree(pb.max_leaf_dofs)
     ctx.set_rhs(pb.rhs)
     maxit, restart = 60, 30
-    prm32 = params_for(ki, pb, 32)
+    prm32 = params_for(ki, pb, 32, predictor=0 if args.no_predictor else 1)
     for w in range(args.warmup):
         tf, ts, it, rel = one_step(ctx, prm32, args.tol, maxit, restart)
         print("[bench] warmup %d: factor %.3f s solve %.3f s iters %d "
@@ -160,6 +192,16 @@ def main():
         print("[bench] step %d: factor %.3f s solve %.3f s iters %d relres "
               "%.2e" % (k, tf, ts, it, rel), file=sys.stderr, flush=True)
     mu32 = ctx.solution()
+    if learned is not None:
+        _, lab, pred = ctx.box_features()
+        ok = (lab >= 0) & (pred >= 0)
+        st = stats[-1]
+        learned.update({
+            "boxes_predicted": st["boxes_predicted"],
+            "boxes_skipped": st["boxes_skipped"],
+            "boxes_fallback": st["boxes_fallback"],
+            "rank_fraction_mae": float(np.mean(np.abs(pred[ok] - lab[ok])))
+            if ok.any() else None})
     tot = [a + b2 for a, b2, _, _ in times]
     ms = 1e3 * float(np.mean(tot))
     value = pb.ndof / (ms * 1e-3)
@@ -213,6 +255,7 @@ def main():
         "phase_s": {"fill": st["t_fill"], "cpqr": st["t_qr"],
                     "decouple": st["t_decouple"], "top": st["t_top"]},
         "plain_fp64": fp64,
+        "learned_rank": learned,
         "kernels": [{"name": k["name"], "launches": k["launches"],
                      "ms": 1e3 * k["seconds"]} for k in clocks[-1]],
         "roofline": roof,

@@ -155,6 +155,15 @@ int ki_lu_factorize(ki_ctx* ctx, const double* a, int n, double* lu,
 int ki_lu_solve(ki_ctx* ctx, const double* lu, const int* piv, int n,
                 double* b, int nrhs, int trans);
 
+/* C = alpha op(A) op(B) + beta C on the device matrix cores (column-major
+ * host arrays; op = transpose when ta / tb = 1; C is m x n, op(A) m x k,
+ * op(B) k x n).  precision 64 computes in FP64, 32 rounds A, B, C to FP32
+ * and computes in FP32 (result returned as FP64).  Replaces the ki_Mat
+ * products (ki_mat.cpp:328-335, cblas_dgemm). */
+int ki_gemm(ki_ctx* ctx, int precision, int ta, int tb, int m, int n, int k,
+            double alpha, const double* a, int lda, const double* b, int ldb,
+            double beta, double* c, int ldc);
+
 /* Factorization statistics. */
 typedef struct ki_stats {
   int levels;

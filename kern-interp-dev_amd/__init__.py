@@ -34,7 +34,7 @@ EXPORTS = [
     "ki_apply_forward", "ki_get_stats", "ki_box_log", "ki_proxy_block",
     "ki_lu_factorize", "ki_lu_solve", "ki_kernel_clock", "ki_box_features",
     "ki_mlp_set_params", "ki_mlp_get_params", "ki_mlp_init", "ki_mlp_grad",
-    "ki_mlp_train", "ki_mlp_predict",
+    "ki_mlp_train", "ki_mlp_predict", "ki_gemm",
 ]
 
 MLP_NFEAT, MLP_NPARAM = 12, 5057
@@ -141,6 +141,9 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "ki_mlp_train": (C.c_int, [vp, fp, fp, C.c_int, C.c_int, C.c_int,
                                    C.c_float, C.c_uint64, fp]),
         "ki_mlp_predict": (C.c_int, [vp, fp, C.c_int, C.c_int, fp]),
+        "ki_gemm": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                              C.c_int, C.c_double, dp, C.c_int, dp, C.c_int,
+                              C.c_double, dp, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -357,6 +360,22 @@ class Context:
         _check(self.lib.ki_lu_solve(self.h, _dptr(lu), _iptr(piv), n,
                                     _dptr(x), nrhs, int(bool(trans))))
         return x
+
+    def gemm(self, a, b, c=None, alpha=1.0, beta=0.0, ta=False, tb=False,
+             precision=64):
+        """alpha op(a) op(b) + beta c on the device (ki_Mat products)."""
+        a = np.asfortranarray(a, dtype=np.float64)
+        b = np.asfortranarray(b, dtype=np.float64)
+        m = a.shape[1] if ta else a.shape[0]
+        k = a.shape[0] if ta else a.shape[1]
+        n = b.shape[0] if tb else b.shape[1]
+        out = np.zeros((m, n), order="F") if c is None else \
+            np.asfortranarray(c, dtype=np.float64).copy(order="F")
+        _check(self.lib.ki_gemm(self.h, precision, int(ta), int(tb), m, n, k,
+                                alpha, _dptr(a), max(a.shape[0], 1), _dptr(b),
+                                max(b.shape[0], 1), beta, _dptr(out),
+                                max(m, 1)))
+        return out
 
     def stats(self) -> dict:
         st = KiStats()

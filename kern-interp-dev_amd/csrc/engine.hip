@@ -902,7 +902,8 @@ void skeletonize_t(Ctx& c) {
         st.flops_schur += 2.0 * gd[q].m * gd[q].n * gd[q].k;
       }
       upload(c.gdesc, gd, s);
-      launch_gemm<T>(c.gdesc.as<GemmDesc>(), ncp, mm, nn, A, B, C, s);
+      launch_gemm<T>(c.gdesc.as<GemmDesc>(), ncp, mm, nn, A, B, C, s,
+                     gd[0].ta, gd[0].tb);
       KI_HIP(hipStreamSynchronize(s));
     };
     // (a) K_rs' = K_rs - T^T K_ss
@@ -1876,6 +1877,55 @@ int ki_lu_solve(ki_ctx* ctx, const double* lu, const int* piv, int n,
     KI_HIP(hipMemcpyAsync(b, dA + (size_t)n * n, bbytes,
                           hipMemcpyDeviceToHost, c.s));
     KI_HIP(hipStreamSynchronize(c.s));
+  });
+}
+
+}  // extern "C"
+
+template <typename T>
+static void gemm_host(ki::Ctx& c, int ta, int tb, int m, int n, int k,
+                      double alpha, const double* a, int lda, const double* b,
+                      int ldb, double beta, double* cm, int ldc) {
+  using namespace ki;
+  const int64_t na = (int64_t)lda * (ta ? m : k);
+  const int64_t nb = (int64_t)ldb * (tb ? k : n);
+  const int64_t nc = (int64_t)ldc * n;
+  std::vector<T> h((size_t)(na + nb + nc));
+  for (int64_t e = 0; e < na; e++) h[e] = (T)a[e];
+  for (int64_t e = 0; e < nb; e++) h[na + e] = (T)b[e];
+  for (int64_t e = 0; e < nc; e++) h[na + nb + e] = (T)cm[e];
+  upload(c.tmp, h, c.s);
+  std::vector<GemmDesc> gd{GemmDesc{m, n, k, 0, na, na + nb, lda, ldb, ldc,
+                                    ta, tb, alpha, beta}};
+  upload(c.gdesc, gd, c.s);
+  T* p = c.tmp.as<T>();
+  launch_gemm<T>(c.gdesc.as<GemmDesc>(), 1, m, n, p, p, p, c.s, ta, tb);
+  KI_HIP(hipMemcpyAsync(h.data() + na + nb, p + na + nb, sizeof(T) * nc,
+                        hipMemcpyDeviceToHost, c.s));
+  KI_HIP(hipStreamSynchronize(c.s));
+  for (int64_t e = 0; e < nc; e++) cm[e] = (double)h[na + nb + e];
+}
+
+extern "C" {
+
+int ki_gemm(ki_ctx* ctx, int precision, int ta, int tb, int m, int n, int k,
+            double alpha, const double* a, int lda, const double* b, int ldb,
+            double beta, double* c, int ldc) {
+  return guard([&] {
+    KI_CHECK(m >= 0 && n >= 0 && k >= 0 && a && b && c, KI_E_ARG,
+             "bad GEMM arguments");
+    KI_CHECK(lda >= std::max(1, ta ? k : m) && ldb >= std::max(1, tb ? n : k) &&
+                 ldc >= std::max(1, m),
+             KI_E_ARG, "leading dimension too small");
+    KI_CHECK(precision == 32 || precision == 64, KI_E_ARG,
+             "precision must be 32 or 64");
+    if (m == 0 || n == 0) return;
+    if (precision == 32)
+      gemm_host<float>(ctx->c, ta ? 1 : 0, tb ? 1 : 0, m, n, k, alpha, a, lda,
+                       b, ldb, beta, c, ldc);
+    else
+      gemm_host<double>(ctx->c, ta ? 1 : 0, tb ? 1 : 0, m, n, k, alpha, a,
+                        lda, b, ldb, beta, c, ldc);
   });
 }
 

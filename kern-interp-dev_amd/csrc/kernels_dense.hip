@@ -8,8 +8,6 @@
 // (ki_mat.cpp:328-335, 445-513) inside decouple (skel_factorization.cpp:
 // 91-127), the top-level LU (:285-310) and the block-diagonal solves of the
 // sweep (:585-592, 676-722).
-#include <hip/hip_cooperative_groups.h>
-
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -111,6 +109,117 @@ gemm_kernel(const GemmDesc* __restrict__ desc, const T* __restrict__ a_all,
     const int gi = i0 + wr + 16 * a + (lane & 15);
 #pragma unroll
     for (int b = 0; b < 2; b++) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int gj = j0 + wc + 16 * b + Mfma<T>::row(lane, v);
+        if (gi >= d.m || gj >= d.n) continue;
+        T* cp = C + gi + (int64_t)d.ldc * gj;
+        const T x = acc[a][b][v];
+        *cp = beta == (T)0 ? alpha * x : alpha * x + beta * *cp;
+      }
+    }
+  }
+}
+
+// The same product on 128 x 128 tiles for large matrices: four waves of
+// 64 x 64 (4 x 4 MFMA tiles), K slices of 16 staged in LDS k-contiguous with
+// a 17-element row stride, and the next slice prefetched into registers
+// while the current one is multiplied.  TA / TB pick the global read order
+// that keeps the loads coalesced for each operand layout.
+constexpr int H = 128, HK = 16, HP = HK + 1;
+
+template <typename T, int TA, int TB>
+__global__ void __launch_bounds__(256)
+gemm128_kernel(const GemmDesc* __restrict__ desc, const T* __restrict__ a_all,
+               const T* __restrict__ b_all, T* __restrict__ c_all) {
+  const GemmDesc d = desc[blockIdx.y];
+  const int tiles_m = (d.m + H - 1) / H;
+  const int tiles_n = (d.n + H - 1) / H;
+  if ((int)blockIdx.x >= tiles_m * tiles_n) return;
+  const int i0 = (blockIdx.x % tiles_m) * H, j0 = (blockIdx.x / tiles_m) * H;
+  const T* A = a_all + d.a_off;
+  const T* B = b_all + d.b_off;
+  T* C = c_all + d.c_off;
+  const int64_t lda = d.lda, ldb = d.ldb;
+  __shared__ T As[H][HP];  // As[i][k] = op(A)(i0 + i, k0 + k)
+  __shared__ T Bs[H][HP];  // Bs[j][k] = op(B)(k0 + k, j0 + j)
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int wr = (wid >> 1) * 64, wc = (wid & 1) * 64;
+  // one global slice covers KL values of k: 128-byte segments along k
+  constexpr int KL = sizeof(T) == 4 ? 32 : 16, NR = H * KL / 256;
+  // element r of a thread's slice: op(A)(i, k) and op(B)(k, j) coordinates.
+  // TA = 0 reads along i (A column-major m x k), TA = 1 along k (A stored
+  // k x m); TB = 0 reads along k (B column-major k x n), TB = 1 along j.
+  auto acoord = [&](int r, int& i, int& k) {
+    const int e = tid + 256 * r;
+    i = TA ? e / KL : e % H;
+    k = TA ? e % KL : e / H;
+  };
+  auto bcoord = [&](int r, int& j, int& k) {
+    const int e = tid + 256 * r;
+    j = TB ? e % H : e / KL;
+    k = TB ? e / H : e % KL;
+  };
+  T ra[NR], rb[NR];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      int ai, ak, bj, bk;
+      acoord(r, ai, ak);
+      bcoord(r, bj, bk);
+      const int gi = i0 + ai, gk = k0 + ak;
+      ra[r] = (gi < d.m && gk < d.k)
+                  ? (TA ? A[gk + lda * gi] : A[gi + lda * gk])
+                  : (T)0;
+      const int gj = j0 + bj, gk2 = k0 + bk;
+      rb[r] = (gj < d.n && gk2 < d.k)
+                  ? (TB ? B[gj + ldb * gk2] : B[gk2 + ldb * gj])
+                  : (T)0;
+    }
+  };
+  typedef typename Mfma<T>::acc_t acc_t;
+  acc_t acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[a][b] = acc_t{0, 0, 0, 0};
+  gload(0);
+  for (int k0 = 0; k0 < d.k; k0 += KL) {
+#pragma unroll
+    for (int h = 0; h < KL / HK; h++) {
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        int ai, ak, bj, bk;
+        acoord(r, ai, ak);
+        bcoord(r, bj, bk);
+        if (ak / HK == h) As[ai][ak % HK] = ra[r];
+        if (bk / HK == h) Bs[bj][bk % HK] = rb[r];
+      }
+      __syncthreads();
+      if (h == KL / HK - 1 && k0 + KL < d.k) gload(k0 + KL);
+#pragma unroll
+      for (int ks = 0; ks < HK; ks += 4) {
+        const int kr = ks + (lane >> 4);
+        T av[4], bv[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) av[a] = As[wr + 16 * a + (lane & 15)][kr];
+#pragma unroll
+        for (int b = 0; b < 4; b++) bv[b] = Bs[wc + 16 * b + (lane & 15)][kr];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+          for (int b = 0; b < 4; b++)
+            acc[a][b] = Mfma<T>::mac(bv[b], av[a], acc[a][b]);
+      }
+    }
+  }
+  const T alpha = (T)d.alpha, beta = (T)d.beta;
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    const int gi = i0 + wr + 16 * a + (lane & 15);
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         const int gj = j0 + wc + 16 * b + Mfma<T>::row(lane, v);
@@ -296,14 +405,44 @@ lu_pstep(T* __restrict__ A, int ld, int m, int j0, int jb,
 // its rows below c.  Posted data goes to per-step slots, so no slot is
 // rewritten while another workgroup may still read it.  Same interchanges as
 // the one-workgroup panel (LAPACK getf2 order).
+//
+// The grid barrier is one arrival counter per launch (zeroed by the host
+// before the panels run): each workgroup adds 1 after its posts (release),
+// then waits for (c + 1) * nwg arrivals (acquire).  The cooperative launch
+// guarantees that every workgroup is resident; the wait is still bounded,
+// and a workgroup that gives up sets *fault so the host reports an error
+// instead of hanging.
 constexpr int PR = 224;  // panel rows per workgroup (LDS: PR x LB values)
+constexpr long long kBarrierSpins = 1ll << 26;
+
+__device__ __forceinline__ void coop_barrier(unsigned* count, unsigned target,
+                                             int* fault) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    long long spins = 0;
+    while (__hip_atomic_load(count, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > kBarrierSpins) {
+        __hip_atomic_store(fault, 1, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
 
 template <typename T>
 __global__ void __launch_bounds__(256)
 lu_coop_panel(T* __restrict__ A, int ld, int m, int j0, int jb,
               int* __restrict__ piv, T* __restrict__ cv, int* __restrict__ ci,
-              T* __restrict__ crow, T* __restrict__ rowc) {
-  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+              T* __restrict__ crow, T* __restrict__ rowc,
+              unsigned* __restrict__ bar, int* __restrict__ fault) {
   const int nwg = gridDim.x, b = blockIdx.x, tid = threadIdx.x;
   const int mr = m - j0;
   const int r0 = b * PR;
@@ -360,9 +499,8 @@ lu_coop_panel(T* __restrict__ A, int ld, int m, int j0, int jb,
       crow[((int64_t)c * nwg + b) * LB + tid] = P[tid][mine - r0];
     if (c >= r0 && c < r0 + nr && tid < jb)
       rowc[(int64_t)c * LB + tid] = P[tid][c - r0];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    grid.sync();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __threadfence();
+    coop_barrier(bar, (unsigned)(c + 1) * nwg, fault);
     // global pivot: the same reduction order in every workgroup
     if (tid < 64) {
       T bb = -1;
@@ -659,13 +797,29 @@ scatter_kernel(const ScatterDesc* __restrict__ desc, const TS* __restrict__ src_
 
 template <typename T>
 void launch_gemm(const GemmDesc* d_desc, int ndesc, int max_m, int max_n,
-                 const T* a, const T* b, T* c, hipStream_t s) {
+                 const T* a, const T* b, T* c, hipStream_t s, int ta, int tb) {
   if (ndesc == 0 || max_m == 0 || max_n == 0) return;
-  const int tiles = ((max_m + GT - 1) / GT) * ((max_n + GT - 1) / GT);
+  const bool big = max_m >= 512 && max_n >= 512;
+  const int tile = big ? H : GT;
+  const int tiles = ((max_m + tile - 1) / tile) * ((max_n + tile - 1) / tile);
   for (int base = 0; base < ndesc; base += 65535) {
     const int cnt = std::min(65535, ndesc - base);
-    hipLaunchKernelGGL(gemm_kernel<T>, dim3(tiles, cnt), dim3(256), 0, s,
-                       d_desc + base, a, b, c);
+    const dim3 g(tiles, cnt);
+    const GemmDesc* dd = d_desc + base;
+    if (!big)
+      hipLaunchKernelGGL(gemm_kernel<T>, g, dim3(256), 0, s, dd, a, b, c);
+    else if (!ta && !tb)
+      hipLaunchKernelGGL((gemm128_kernel<T, 0, 0>), g, dim3(256), 0, s, dd, a,
+                         b, c);
+    else if (ta && !tb)
+      hipLaunchKernelGGL((gemm128_kernel<T, 1, 0>), g, dim3(256), 0, s, dd, a,
+                         b, c);
+    else if (!ta && tb)
+      hipLaunchKernelGGL((gemm128_kernel<T, 0, 1>), g, dim3(256), 0, s, dd, a,
+                         b, c);
+    else
+      hipLaunchKernelGGL((gemm128_kernel<T, 1, 1>), g, dim3(256), 0, s, dd, a,
+                         b, c);
   }
   KI_HIP(hipGetLastError());
 }
@@ -729,13 +883,18 @@ static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
   }
   T *cv = nullptr, *crow = nullptr, *rowc = nullptr;
   int* ci = nullptr;
+  unsigned* bar = nullptr;  // one arrival counter per panel, then the fault
+  const int npanels = (mn + LB - 1) / LB;
   if (coop) {
     KI_HIP(hipMallocAsync((void**)&cv, sizeof(T) * (size_t)LB * coop_wg, s));
     KI_HIP(hipMallocAsync((void**)&ci, sizeof(int) * (size_t)LB * coop_wg, s));
     KI_HIP(hipMallocAsync((void**)&crow,
                           sizeof(T) * (size_t)LB * coop_wg * LB, s));
     KI_HIP(hipMallocAsync((void**)&rowc, sizeof(T) * (size_t)LB * LB, s));
+    KI_HIP(hipMallocAsync((void**)&bar, sizeof(unsigned) * (npanels + 1), s));
+    KI_HIP(hipMemsetAsync(bar, 0, sizeof(unsigned) * (npanels + 1), s));
   }
+  int* fault = coop ? reinterpret_cast<int*>(bar + npanels) : nullptr;
   size_t gi = 0;
   auto cols = [](int c) { return dim3((std::max(c, 1) + 255) / 256); };
   for (int J0 = 0; J0 < mn; J0 += NB) {
@@ -746,8 +905,9 @@ static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
       const dim3 g((mr + 255) / 256);
       if (coop) {
         int a_ld = ld, a_m = m, a_j0 = j0, a_jb = jb;
-        void* args[] = {&A,  &a_ld, &a_m, &a_j0, &a_jb, &piv,
-                        &cv, &ci,   &crow, &rowc};
+        unsigned* pbar = bar + j0 / LB;
+        void* args[] = {&A,  &a_ld, &a_m,  &a_j0, &a_jb, &piv,
+                        &cv, &ci,   &crow, &rowc, &pbar, &fault};
         KI_HIP(hipLaunchCooperativeKernel(
             reinterpret_cast<const void*>(&lu_coop_panel<T>),
             dim3((mr + PR - 1) / PR), dim3(256), args, 0, s));
@@ -769,7 +929,7 @@ static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
         hipLaunchKernelGGL(trsm_small_cols<T>, cols(c1 - c0), dim3(256), 0, s,
                            A, ld, j0, jb, c0, c1);
         const GemmDesc& q = gd[gi];
-        launch_gemm<T>(d_g + gi, 1, q.m, q.n, a, a, a, s);
+        launch_gemm<T>(d_g + gi, 1, q.m, q.n, a, a, a, s, 0, 0);
         gi++;
       }
     }
@@ -794,7 +954,7 @@ static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
       trsm_sweeps<T>(hv, d_m, a, nullptr, a, mode0, 1, 0, s);
       if (m > J0 + JB) {
         const GemmDesc& q = gd[gi];
-        launch_gemm<T>(d_g + gi, 1, q.m, q.n, a, a, a, s);
+        launch_gemm<T>(d_g + gi, 1, q.m, q.n, a, a, a, s, 0, 0);
         gi++;
       }
     }
@@ -809,6 +969,11 @@ static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
     KI_HIP(hipFreeAsync(ci, s));
     KI_HIP(hipFreeAsync(crow, s));
     KI_HIP(hipFreeAsync(rowc, s));
+    int hf = 0;
+    KI_HIP(hipMemcpyAsync(&hf, fault, sizeof(int), hipMemcpyDeviceToHost, s));
+    KI_HIP(hipStreamSynchronize(s));
+    KI_HIP(hipFreeAsync(bar, s));
+    KI_CHECK(hf == 0, 3, "LU panel grid barrier timed out");  // device error
   }
 }
 
@@ -869,7 +1034,8 @@ void batched_lu(const std::vector<MatDesc>& h, MatDesc* d_desc, T* a, int* piv,
     hipLaunchKernelGGL(lu_swap_trsm<T>, dim3((maxn + 255) / 256, nb),
                        dim3(256), 0, s, d_desc, a, piv, j0);
     if (mm[p] > 0 && nn[p] > 0)
-      launch_gemm<T>(d_g + (size_t)p * nb, nb, mm[p], nn[p], a, a, a, s);
+      launch_gemm<T>(d_g + (size_t)p * nb, nb, mm[p], nn[p], a, a, a, s, 0,
+                     0);
   }
   KI_HIP(hipGetLastError());
   // the host descriptor vector dies here: make sure the upload has landed
@@ -959,7 +1125,7 @@ static void trsm_sweeps(const std::vector<MatDesc>& h, MatDesc* d_desc,
                          dim3(TC), 0, s, d_desc, a, b, blk, modes[mi]);
       if (mm[slot] > 0 && nn[slot] > 0)
         launch_gemm<T>(d_g + (size_t)slot * nb, nb, mm[slot], nn[slot], a, b,
-                       b, s);
+                       b, s, modes[mi] >= 2 ? 1 : 0, 0);
     }
   }
   if (swap == 2)
@@ -1036,7 +1202,7 @@ void launch_scatter(const ScatterDesc* d_desc, int ndesc, int max_n,
 
 #define KI_INST(T)                                                            \
   template void launch_gemm<T>(const GemmDesc*, int, int, int, const T*,      \
-                               const T*, T*, hipStream_t);                    \
+                               const T*, T*, hipStream_t, int, int);          \
   template void batched_lu<T>(const std::vector<MatDesc>&, MatDesc*, T*,      \
                               int*, hipStream_t);                             \
   template void batched_trsm_upper<T>(const std::vector<MatDesc>&, MatDesc*,  \

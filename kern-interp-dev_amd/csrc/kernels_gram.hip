@@ -99,6 +99,99 @@ gram_kernel(const GramDesc* __restrict__ desc, const TI* __restrict__ a_all,
   }
 }
 
+// The same product on 128 x 128 tiles for large boxes: four waves of 64 x 64
+// (4 x 4 MFMA tiles each), K slices of 16 staged in LDS k-contiguous with a
+// 17-element row stride (conflict-free stores and operand reads), and the
+// next slice prefetched into registers while the current one is multiplied.
+constexpr int G2 = 128, G2K = 16, G2P = G2K + 1;
+
+template <typename TI>
+__global__ void __launch_bounds__(256)
+gram128_kernel(const GramDesc* __restrict__ desc, const TI* __restrict__ a_all,
+               double* __restrict__ g_all) {
+  const GramDesc d = desc[blockIdx.y];
+  const int tn = (d.n + G2 - 1) / G2;
+  const int t = blockIdx.x;
+  if (t >= tn * (tn + 1) / 2) return;
+  int tj = (int)((sqrt(8.0 * t + 1.0) - 1.0) / 2.0);
+  while ((tj + 1) * (tj + 2) / 2 <= t) tj++;
+  while (tj * (tj + 1) / 2 > t) tj--;
+  const int ti = t - tj * (tj + 1) / 2;
+  const int i0 = ti * G2, j0 = tj * G2;
+  const TI* A = a_all + d.a_off;
+  const int m = d.m, n = d.n;
+  const int64_t lda = d.lda;
+  __shared__ double As[G2][G2P];  // As[i][k] = A(k0 + k, i0 + i)
+  __shared__ double Bs[G2][G2P];  // Bs[j][k] = A(k0 + k, j0 + j)
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int wr = (wid >> 1) * 64, wc = (wid & 1) * 64;
+  // one global slice covers KL rows of A: 128 bytes per column segment
+  constexpr int KL = sizeof(TI) == 4 ? 32 : 16, NR = G2 * KL / 256;
+  TI ra[NR], rb[NR];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      const int e = tid + 256 * r;
+      const int k = e % KL, i = e / KL;
+      const int gk = k0 + k, gi = i0 + i, gj = j0 + i;
+      ra[r] = (gk < m && gi < n) ? A[gk + lda * gi] : (TI)0;
+      rb[r] = (gk < m && gj < n) ? A[gk + lda * gj] : (TI)0;
+    }
+  };
+  f64x4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[a][b] = f64x4{0, 0, 0, 0};
+  gload(0);
+  for (int k0 = 0; k0 < m; k0 += KL) {
+#pragma unroll
+    for (int h = 0; h < KL / G2K; h++) {
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        const int e = tid + 256 * r;
+        const int k = e % KL, i = e / KL;
+        if (k / G2K == h) {
+          As[i][k % G2K] = (double)ra[r];
+          Bs[i][k % G2K] = (double)rb[r];
+        }
+      }
+      __syncthreads();
+      if (h == KL / G2K - 1 && k0 + KL < m) gload(k0 + KL);
+#pragma unroll
+      for (int ks = 0; ks < G2K; ks += 4) {
+        const int kr = ks + (lane >> 4);
+        double av[4], bv[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) av[a] = As[wr + 16 * a + (lane & 15)][kr];
+#pragma unroll
+        for (int b = 0; b < 4; b++) bv[b] = Bs[wc + 16 * b + (lane & 15)][kr];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+          for (int b = 0; b < 4; b++)
+            acc[a][b] = Mfma<double>::mac(bv[b], av[a], acc[a][b]);
+      }
+    }
+  }
+  double* G = g_all + d.g_off;
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    const int gi = i0 + wr + 16 * a + (lane & 15);
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int gj = j0 + wc + 16 * b + Mfma<double>::row(lane, v);
+        if (gi >= n || gj >= n) continue;
+        G[gi + (int64_t)n * gj] = acc[a][b][v];
+        G[gj + (int64_t)n * gi] = acc[a][b][v];
+      }
+    }
+  }
+}
+
 // Argmax of the Schur diagonal over unselected columns (selected columns
 // carry the sentinel -1; the first index wins ties, as idamax).
 __device__ __forceinline__ void argmax_dev(const double* dg, int n,
@@ -224,7 +317,9 @@ pchol_wg(const CholDesc* __restrict__ desc, double* __restrict__ g_all,
 // S(:, p) and the block's own columns i0..i-1.  The Schur diagonal is
 // double-buffered (din / dout) so that every workgroup of a step reads the
 // same values and picks the same pivot.
-constexpr int CB = 32;
+// CB = 128: the trailing update rewrites all of S once per block, so a wide
+// block cuts that HBM traffic; each step's dot products grow to < CB terms.
+constexpr int CB = 128;
 
 __global__ void __launch_bounds__(256)
 pchol_step(const CholDesc* __restrict__ desc, const double* __restrict__ g_all,
@@ -334,12 +429,18 @@ template <typename TI>
 void launch_gram(const GramDesc* d_desc, int ndesc, int max_n, const TI* a,
                  double* g, hipStream_t s) {
   if (ndesc == 0 || max_n == 0) return;
-  const int tn = (max_n + GT - 1) / GT;
+  const bool big = max_n >= 512;
+  const int tile = big ? G2 : GT;
+  const int tn = (max_n + tile - 1) / tile;
   const int tiles = tn * (tn + 1) / 2;
   for (int base = 0; base < ndesc; base += 65535) {
     const int cnt = std::min(65535, ndesc - base);
-    hipLaunchKernelGGL(gram_kernel<TI>, dim3(tiles, cnt), dim3(256), 0, s,
-                       d_desc + base, a, g);
+    if (big)
+      hipLaunchKernelGGL(gram128_kernel<TI>, dim3(tiles, cnt), dim3(256), 0,
+                         s, d_desc + base, a, g);
+    else
+      hipLaunchKernelGGL(gram_kernel<TI>, dim3(tiles, cnt), dim3(256), 0, s,
+                         d_desc + base, a, g);
   }
   KI_HIP(hipGetLastError());
 }
@@ -395,7 +496,7 @@ void pchol_large(const std::vector<CholDesc>& h, const CholDesc* d_desc,
                        state, i, i0, i & 1, tol);
     if (i % CB == CB - 1 && i + 1 < maxsteps)
       launch_gemm<double>(d_g + (size_t)(i / CB) * nb, nb, maxn, maxn, ws, ws,
-                          g, s);
+                          g, s, 0, 1);
     if ((i & 63) == 63) {  // stop early once every box has met its rank
       KI_HIP(hipMemcpyAsync(hst.data(), state, sizeof(int) * 4 * nb,
                             hipMemcpyDeviceToHost, s));

@@ -211,9 +211,11 @@ struct GemmDesc {
   double alpha, beta;
 };
 
+// ta / tb: the (common) transposition flags of every descriptor in the
+// launch; they select the 128 x 128-tile kernel for large products.
 template <typename T>
 void launch_gemm(const GemmDesc* d_desc, int ndesc, int max_m, int max_n,
-                 const T* a, const T* b, T* c, hipStream_t s);
+                 const T* a, const T* b, T* c, hipStream_t s, int ta, int tb);
 
 struct MatDesc {       // one square (or m x n) matrix for LU / TRSM / copy
   int m, n;

@@ -348,3 +348,21 @@ def test_dense_lu_solve_matches_lapack(n, nrhs, trans):
     x = ctx.lu_solve(lu_ref, piv_ref + 1, b, trans=trans)
     xr = sla.lu_solve((lu_ref, piv_ref), b, trans=1 if trans else 0)
     assert np.abs(x - xr).max() <= 1e-10 * np.abs(xr).max()
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("m,n,k", [(70, 45, 33), (1030, 777, 300)])
+def test_gemm_matches_numpy(m, n, k, ta, tb, precision):
+    """Both tile sizes (64 below 512 rows/cols, 128 above) and every operand
+    layout; FP64 to 1e-13, FP32 to 1e-5 relative (k-term sums in FP32)."""
+    rng = np.random.default_rng(m + 7 * n + 3 * k + 11 * ta + 5 * tb)
+    a = rng.standard_normal((k, m) if ta else (m, k))
+    b = rng.standard_normal((n, k) if tb else (k, n))
+    c = rng.standard_normal((m, n))
+    ctx = ki.Context()
+    got = ctx.gemm(a, b, c, alpha=-0.75, beta=1.25, ta=ta, tb=tb,
+                   precision=precision)
+    want = -0.75 * (a.T if ta else a) @ (b.T if tb else b) + 1.25 * c
+    tol = 1e-13 if precision == 64 else 1e-5
+    assert np.linalg.norm(got - want) <= tol * np.linalg.norm(want)
