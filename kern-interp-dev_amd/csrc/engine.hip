@@ -530,19 +530,20 @@ void skeletonize_t(Ctx& c) {
       auto run_id = [&](const std::vector<int>& sel) {
         std::vector<FillDesc> sf;
         std::vector<QrDesc> sq;
-        int mc = 0;
+        int mc = 0, mr = 0;
         double el = 0.;
         for (int q : sel) {
           sf.push_back(fd[q]);
           sq.push_back(qd[q]);
           mc = std::max(mc, qd[q].n);
+          mr = std::max(mr, qd[q].m);
           el += (double)qd[q].m * qd[q].n;
         }
         const int ns = sel.size();
         upload(c.fdesc, sf, s);
         {
           Timer tm(s);
-          launch_fill<T>(c.g, px, c.fdesc.as<FillDesc>(), ns, mc,
+          launch_fill<T>(c.g, px, c.fdesc.as<FillDesc>(), ns, mc, mr,
                          c.idxb.as<int>(), c.idm.as<T>(), s);
           const double dt = tm.stop();
           st.t_fill += dt;
@@ -817,7 +818,7 @@ void skeletonize_t(Ctx& c) {
     T* K = c.kbn.as<T>();
     upload(c.idxb, bidx, s);
     upload(c.fdesc, kf, s);
-    launch_fill<T>(c.g, px, c.fdesc.as<FillDesc>(), ncp, maxn,
+    launch_fill<T>(c.g, px, c.fdesc.as<FillDesc>(), ncp, maxn, maxn,
                    c.idxb.as<int>(), K, s);
     // subtract children's accumulated updates; seed Acc with their skel part
     {
@@ -1044,7 +1045,7 @@ void skeletonize_t(Ctx& c) {
     f.ld = nt;
     std::vector<FillDesc> fv{f};
     upload(c.fdesc, fv, s);
-    launch_fill<double>(c.g, px, c.fdesc.as<FillDesc>(), 1, nt,
+    launch_fill<double>(c.g, px, c.fdesc.as<FillDesc>(), 1, nt, nt,
                         c.d_allskel.as<int>(), c.top.as<double>(), s);
     if (L > 1) {
       std::vector<ScatterDesc> sd;
@@ -1748,7 +1749,7 @@ int ki_kernel_block(ki_ctx* ctx, const int* tgt, int nt, const int* src,
     ki::upload(c.fdesc, fv, c.s);
     c.tmp.ensure(sizeof(double) * (size_t)nt * ns);
     ki::ProxySpec px{};
-    ki::launch_fill<double>(c.g, px, c.fdesc.as<ki::FillDesc>(), 1, ns,
+    ki::launch_fill<double>(c.g, px, c.fdesc.as<ki::FillDesc>(), 1, ns, nt,
                             c.idxb.as<int>(), c.tmp.as<double>(), c.s);
     KI_HIP(hipMemcpyAsync(out, c.tmp.p, sizeof(double) * (size_t)nt * ns,
                           hipMemcpyDeviceToHost, c.s));
@@ -1786,7 +1787,7 @@ int ki_proxy_block(ki_ctx* ctx, const ki_params* params, const double* center,
     std::vector<ki::FillDesc> fv{f};
     ki::upload(c.fdesc, fv, c.s);
     c.tmp.ensure(sizeof(double) * (size_t)P * n);
-    ki::launch_fill<double>(c.g, px, c.fdesc.as<ki::FillDesc>(), 1, n,
+    ki::launch_fill<double>(c.g, px, c.fdesc.as<ki::FillDesc>(), 1, n, P,
                             c.idxb.as<int>(), c.tmp.as<double>(), c.s);
     KI_HIP(hipMemcpyAsync(out, c.tmp.p, sizeof(double) * (size_t)P * n,
                           hipMemcpyDeviceToHost, c.s));

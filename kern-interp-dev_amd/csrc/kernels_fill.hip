@@ -129,53 +129,69 @@ __device__ __forceinline__ T entry(const DevGeom& g, const Pt& t, int a,
          (T)s.w * (T)t.n[a] * (T)s.n[b];
 }
 
+// One workgroup fills a tile of FR rows x FC columns of one block: each
+// thread resolves its row once (segment, DOF, point or proxy point), the
+// tile's column points sit in LDS, and the thread then produces its row's
+// FC entries; for every column the 256 threads store 256 consecutive rows.
+constexpr int FR = 256, FC = 16;
+
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(FR)
 fill_kernel(DevGeom g, ProxySpec px, const FillDesc* __restrict__ desc,
             const int* __restrict__ idx, T* __restrict__ out) {
-  const FillDesc& d = desc[blockIdx.y];
-  const int j = blockIdx.x;
-  if (j >= d.ncols) return;
+  const FillDesc& d = desc[blockIdx.z];
+  const int c0 = blockIdx.x * FC;
+  if (c0 >= d.ncols) return;
+  int m = 0;
+  for (int sg = 0; sg < d.nseg; sg++) m += d.seg_rows[sg];
+  const int i = blockIdx.y * FR + threadIdx.x;
+  if ((int)blockIdx.y * FR >= m) return;
   const int sol = g.sol;
-  const int cdof = idx[d.col_idx + j];
-  const int cp = cdof / sol, cpar = cdof % sol;
-  Pt cpt;
-  load_pt(g, cp, cpt);
-  T* col = out + d.out_off + (int64_t)d.ld * j;
-  int row0 = 0;
-  for (int sgi = 0; sgi < d.nseg; sgi++) {
-    const int kind = d.seg_kind[sgi];
-    const int nr = d.seg_rows[sgi];
-    const int* rl = idx + d.seg_idx[sgi];
-    for (int i = threadIdx.x; i < nr; i += blockDim.x) {
-      T v;
-      if (kind == SEG_K) {
-        const int rd = rl[i];
-        Pt tp;
-        load_pt(g, rd / sol, tp);
-        v = entry<T>(g, tp, rd % sol, cpt, cpar, cp, true);
-      } else if (kind == SEG_KT) {
-        const int rd = rl[i];
-        Pt sp;
-        load_pt(g, rd / sol, sp);
-        v = entry<T>(g, cpt, cpar, sp, rd % sol, rd / sol, true);
-      } else {
-        const int half = nr / 2;
-        Pt pp;
-        if (i < half) {  // box -> proxy: target proxy point, source column DOF
-          proxy_pt(px, g.dom, d.center, d.radius, i / sol, pp);
-          Pt src = cpt;
-          src.c = 0.0;
-          v = entry<T>(g, pp, i % sol, src, cpar, cp, false);
-        } else {         // proxy -> box: target column DOF, source proxy point
-          const int q = i - half;
-          proxy_pt(px, g.dom, d.center, d.radius, q / sol, pp);
-          v = entry<T>(g, cpt, cpar, pp, q % sol, 0, false);
-        }
-      }
-      col[row0 + i] = v;
+  const int nc = min(FC, d.ncols - c0);
+  __shared__ Pt cpts[FC];
+  __shared__ int cdofs[FC];
+  if ((int)threadIdx.x < nc) {
+    const int cd = idx[d.col_idx + c0 + threadIdx.x];
+    cdofs[threadIdx.x] = cd;
+    load_pt(g, cd / sol, cpts[threadIdx.x]);
+  }
+  __syncthreads();
+  if (i >= m) return;
+  int sg = 0, base = 0;
+  while (i >= base + d.seg_rows[sg]) base += d.seg_rows[sg++];
+  const int kind = d.seg_kind[sg], ii = i - base, nr = d.seg_rows[sg];
+  Pt rp;
+  int rcomp, rpt = 0;
+  bool to_proxy = false;
+  if (kind == SEG_K || kind == SEG_KT) {
+    const int rd = idx[d.seg_idx[sg] + ii];
+    rpt = rd / sol;
+    rcomp = rd % sol;
+    load_pt(g, rpt, rp);
+  } else {
+    const int half = nr / 2;
+    to_proxy = ii < half;
+    const int q = to_proxy ? ii : ii - half;
+    proxy_pt(px, g.dom, d.center, d.radius, q / sol, rp);
+    rcomp = q % sol;
+  }
+  T* row = out + d.out_off + i;
+  for (int jj = 0; jj < nc; jj++) {
+    const int cd = cdofs[jj];
+    const int cpi = cd / sol, cpar = cd % sol;
+    T v;
+    if (kind == SEG_K) {
+      v = entry<T>(g, rp, rcomp, cpts[jj], cpar, cpi, true);
+    } else if (kind == SEG_KT) {
+      v = entry<T>(g, cpts[jj], cpar, rp, rcomp, rpt, true);
+    } else if (to_proxy) {  // box -> proxy: target proxy, source column DOF
+      Pt src = cpts[jj];
+      src.c = 0.0;
+      v = entry<T>(g, rp, rcomp, src, cpar, cpi, false);
+    } else {                // proxy -> box: target column DOF, source proxy
+      v = entry<T>(g, cpts[jj], cpar, rp, rcomp, 0, false);
     }
-    row0 += nr;
+    row[(int64_t)d.ld * (c0 + jj)] = v;
   }
 }
 
@@ -183,21 +199,22 @@ fill_kernel(DevGeom g, ProxySpec px, const FillDesc* __restrict__ desc,
 
 template <typename T>
 void launch_fill(const DevGeom& g, const ProxySpec& px, const FillDesc* d_desc,
-                 int ndesc, int max_cols, const int* d_idx, T* out,
-                 hipStream_t s) {
-  if (ndesc == 0 || max_cols == 0) return;
+                 int ndesc, int max_cols, int max_rows, const int* d_idx,
+                 T* out, hipStream_t s) {
+  if (ndesc == 0 || max_cols == 0 || max_rows == 0) return;
   KI_CHECK(ndesc <= 65535, 2, "too many blocks in one fill launch");
-  dim3 grid(max_cols, ndesc);
-  hipLaunchKernelGGL(fill_kernel<T>, grid, dim3(256), 0, s, g, px, d_desc,
+  KI_CHECK((max_rows + FR - 1) / FR <= 65535, 2, "block too tall to fill");
+  dim3 grid((max_cols + FC - 1) / FC, (max_rows + FR - 1) / FR, ndesc);
+  hipLaunchKernelGGL(fill_kernel<T>, grid, dim3(FR), 0, s, g, px, d_desc,
                      d_idx, out);
   KI_HIP(hipGetLastError());
 }
 
 template void launch_fill<float>(const DevGeom&, const ProxySpec&,
-                                 const FillDesc*, int, int, const int*,
+                                 const FillDesc*, int, int, int, const int*,
                                  float*, hipStream_t);
 template void launch_fill<double>(const DevGeom&, const ProxySpec&,
-                                  const FillDesc*, int, int, const int*,
+                                  const FillDesc*, int, int, int, const int*,
                                   double*, hipStream_t);
 
 }  // namespace ki

@@ -8,7 +8,7 @@
 // launch, split into work items of a few dozen outputs per workgroup.  L is
 // stored transposed (r x k) so that every phase reads its factor with
 // coalesced loads.  The diagonal and top-level LU solves are blocked
-// (256-row diagonal blocks, multi-workgroup updates of the other rows).
+// (64-row diagonal blocks, multi-workgroup updates of the other rows).
 //
 // The matvec is the FP64 residual operator K mu for the whole boundary (the
 // reference forms it densely in solve_err, linear_solve.cpp:339-426; here it
@@ -145,9 +145,11 @@ __global__ void diag_scatter_kernel(const TW* __restrict__ w,
 }
 
 // Diagonal block b (forward: b-th from the top of the unit-lower L; backward:
-// b-th from the bottom of the upper U), one 64-lane workgroup per box.
+// b-th from the bottom of the upper U), one workgroup per box: four waves
+// stage the block in LDS, then one wave solves it with lane t holding
+// w[r0 + t] and lane shuffles broadcasting each solved value.
 template <typename T>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(256)
 trsv_diag_kernel(const DiagDesc* __restrict__ desc, T* const* __restrict__ fac,
                  T* __restrict__ w_all, int b, int upper) {
   const DiagDesc d = desc[blockIdx.x];
@@ -157,29 +159,28 @@ trsv_diag_kernel(const DiagDesc* __restrict__ desc, T* const* __restrict__ fac,
   const int r0 = bb * DB, rb = min(DB, r - r0);
   const T* X = fac[d.buf] + d.offX;
   T* w = w_all + d.w_off;
-  __shared__ T ws[DB];
   __shared__ T D[DB][DB + 1];  // D[t][q] = X(r0 + t, r0 + q)
-  const int t = threadIdx.x;
-  for (int q = 0; q < rb; q++)
-    if (t < rb) D[t][q] = X[(r0 + t) + (int64_t)r * (r0 + q)];
-  if (t < rb) ws[t] = w[r0 + t];
+  const int t = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int q = wv; q < rb; q += 4)
+    D[t][q] = t < rb ? X[(r0 + t) + (int64_t)r * (r0 + q)] : (T)0;
   __syncthreads();
+  if (wv != 0) return;
+  T x = t < rb ? w[r0 + t] : (T)0;
   if (!upper) {
     for (int q = 0; q < rb; q++) {
-      const T v = ws[q];
-      if (t > q && t < rb) ws[t] -= D[t][q] * v;
-      __syncthreads();
+      const T v = __shfl(x, q, 64);
+      if (t > q) x -= D[t][q] * v;
     }
   } else {
     for (int q = rb - 1; q >= 0; q--) {
-      const T v = ws[q] / D[q][q];
-      __syncthreads();
-      if (t == q) ws[q] = v;
-      if (t < q) ws[t] -= D[t][q] * v;
-      __syncthreads();
+      const T v = __shfl(x, q, 64) / D[q][q];
+      if (t == q)
+        x = v;
+      else if (t < q)
+        x -= D[t][q] * v;
     }
   }
-  if (t < rb) w[r0 + t] = ws[t];
+  if (t < rb) w[r0 + t] = x;
 }
 
 // Rows outside diagonal block b: forward subtracts L(rows below, block) w_b,
@@ -212,6 +213,16 @@ trsv_update_kernel(const DiagDesc* __restrict__ desc,
 // FP64 matrix-free operator y = K mu over the whole boundary.
 // ---------------------------------------------------------------------------
 constexpr int MT = 256;
+
+// 1 / sqrt(x) for x > 0 in FP64: the hardware estimate (v_rsq_f64) refined
+// by two Newton-Raphson steps y <- y (3 - x y^2) / 2, each of which squares
+// the relative error, so the result is at FP64 rounding level.
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  y = y * (1.5 - 0.5 * x * y * y);
+  y = y * (1.5 - 0.5 * x * y * y);
+  return y;
+}
 
 // Targets are the boundary points themselves (self = 1, the residual
 // operator) or arbitrary domain points with zero normal (self = 0, the
@@ -256,7 +267,7 @@ matvec_kernel(DevGeom g, const double* __restrict__ tpts,
         if (dom == 2)
           acc[0] += -w * (1.0 / (2 * PI)) * dot / rr * sm[q][0];
         else {
-          const double ir = 1.0 / sqrt(rr);
+          const double ir = rsqrt_nr(rr);
           acc[0] += -w * (1.0 / (4 * PI)) * dot * (ir * ir * ir) * sm[q][0];
         }
       } else {
@@ -267,7 +278,7 @@ matvec_kernel(DevGeom g, const double* __restrict__ tpts,
         if (dom == 2) {
           f = w * (1 / PI) * dot / (rr * rr);
         } else {
-          const double ir = 1.0 / sqrt(rr);
+          const double ir = rsqrt_nr(rr);
           const double ir2 = ir * ir;
           f = w * (-3.0 / (4.0 * PI)) * dot * (ir2 * ir2 * ir);
         }
@@ -346,7 +357,7 @@ __global__ void __launch_bounds__(MT) diag3d_kernel(DevGeom g) {
       const double rr = r0 * r0 + r1 * r1 + r2 * r2;
       const double dot = r0 * sn[q][0] + r1 * sn[q][1] + r2 * sn[q][2];
       const double w = sw[q];
-      const double ir = 1.0 / sqrt(rr);
+      const double ir = rsqrt_nr(rr);
       if (g.sol == 1) {
         s[0] += -w * (1.0 / (4 * PI)) * dot * (ir * ir * ir);
       } else {
@@ -421,7 +432,7 @@ void launch_diag_blocked(const DiagDesc* d, int nd, int max_r,
   const int chunks = (max_r + 255) / 256;
   for (int upper = 0; upper < 2; upper++)
     for (int b = 0; b < nb; b++) {
-      hipLaunchKernelGGL(trsv_diag_kernel<TW>, dim3(nd), dim3(64), 0, s, d,
+      hipLaunchKernelGGL(trsv_diag_kernel<TW>, dim3(nd), dim3(256), 0, s, d,
                          facbufs, w, b, upper);
       if (b + 1 < nb)
         hipLaunchKernelGGL(trsv_update_kernel<TW>, dim3(chunks, nd),

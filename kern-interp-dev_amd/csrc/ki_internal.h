@@ -126,10 +126,11 @@ struct FillDesc {
   double radius;           // proxy radius (SEG_PROXY)
 };
 
+// max_cols / max_rows: the largest block of the launch.
 template <typename T>
 void launch_fill(const DevGeom& g, const ProxySpec& px, const FillDesc* d_desc,
-                 int ndesc, int max_cols, const int* d_idx, T* out,
-                 hipStream_t s);
+                 int ndesc, int max_cols, int max_rows, const int* d_idx,
+                 T* out, hipStream_t s);
 
 // Scatter-add jobs: dst[map[i] + ld*map[j]] += sign * src[i + n*j] for an
 // n x n source; with acc, entries whose mapped row and column are both
