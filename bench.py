@@ -252,6 +252,7 @@ def main():
         "relres": worst_rel,
         "setup_s": t_setup,
         "top_size": st["top_size"],
+        "device_bytes": st["device_bytes"],
         "phase_s": {"fill": st["t_fill"], "cpqr": st["t_qr"],
                     "decouple": st["t_decouple"], "top": st["t_top"]},
         "plain_fp64": fp64,

@@ -52,3 +52,14 @@ def test_error_path_without_device_is_reported_not_fatal():
     rc = lib.ki_ctx_create(10 ** 6, ctypes.byref(h))
     assert rc != 0
     assert lib.ki_last_error().decode() != ""
+
+
+def test_cpp_facade_compiles_against_the_c_abi():
+    """include/kern_interp_amd.hpp (the reference-named C++ classes) and the
+    demo caller compile with the host compiler alone."""
+    import subprocess
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall",
+                        "-I", str(ROOT / "include"),
+                        str(ROOT / "tests" / "cpp" / "facade_demo.cpp")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

@@ -1,0 +1,24 @@
+"""The C++ facade drives the GPU solver like a reference caller would.
+
+tests/cpp/facade_demo.cpp (built by __graft_entry__.build()) runs
+boundary_integral_solve and a refined solve through kern_interp_amd.hpp on a
+2D Laplace ellipse problem and checks the interior field against the exact
+harmonic function (1e-8) and the dense residual of the density (1e-10)."""
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+
+def test_facade_demo_solves_the_ellipse_problem():
+    exe = ROOT / "kern-interp-dev_amd" / "_build" / "facade_demo"
+    assert exe.exists(), "run __graft_entry__.build() first"
+    print("[case] facade_demo", flush=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    assert "field_err_refined" in r.stdout
