@@ -56,7 +56,11 @@ typedef struct ki_params {
                                pivot rule; auto picks 2 when tol >= 1e-7)     */
   double fp32_tol_floor;    /* FP32 mode: ID tolerance = max(id_tol, floor);
                                FP64 GMRES restores the requested residual     */
-  int reserved[8];
+  int lu_wide_rows;         /* a single LU taller than this uses the two-level
+                               blocked form (0: library default, 4096)        */
+  int lu_coop_rows;         /* batched LUs taller than this factor each panel
+                               over many workgroups (0: default, 1024)        */
+  int reserved[6];
 } ki_params;
 
 void ki_params_default(ki_params* p);

@@ -52,7 +52,9 @@ class KiParams(C.Structure):
         ("use_predictor", C.c_int),
         ("id_method", C.c_int),
         ("fp32_tol_floor", C.c_double),
-        ("reserved", C.c_int * 8),
+        ("lu_wide_rows", C.c_int),
+        ("lu_coop_rows", C.c_int),
+        ("reserved", C.c_int * 6),
     ]
 
 

@@ -256,6 +256,8 @@ void id_rows(const Ctx& c, const Box& bx, std::vector<int>& rows,
 // + slack steps, and a box whose rank rule has not fired by then is redone
 // with the full factorization.
 constexpr float kSkipFraction = 0.97f;
+// Largest box whose pivoted Cholesky runs in one workgroup (pchol_wg).
+constexpr int kCholOneWorkgroupMax = 128;
 constexpr double kRankMargin = 1.15;
 constexpr int kRankSlack = 16;
 
@@ -576,7 +578,9 @@ void skeletonize_t(Ctx& c) {
           gd.push_back(GramDesc{qd[q].m, n, qd[q].a_off, qd[q].m, gofs[q]});
           CholDesc ch{n,       qd[q].max_rank, q, gofs[q], wof[q],
                       wof[q] + (int64_t)n * n, qd[q].piv_off};
-          (n > 640 ? large : small).push_back(ch);
+          // one workgroup per box only for small boxes; above that the
+          // step-per-launch form spreads each box over n / 64 workgroups
+          (n > kCholOneWorkgroupMax ? large : small).push_back(ch);
         }
         upload(c.gdesc, gd, s);
         {
@@ -1629,8 +1633,14 @@ int ki_skeletonize(ki_ctx* ctx, const ki_params* params) {
              "precision must be 32 or 64");
     KI_CHECK(c.prm.id_tol > 0, KI_E_ARG, "id_tol must be positive");
     KI_CHECK(c.prm.proxy_points > 0, KI_E_ARG, "proxy_points must be > 0");
+    KI_CHECK(c.prm.lu_wide_rows >= 0 && c.prm.lu_coop_rows >= 0, KI_E_ARG,
+             "LU row thresholds must be >= 0");
     c.factored = false;
     c.prec = c.prm.precision;
+    struct Thresholds {  // this call's LU thresholds, defaults afterwards
+      Thresholds(int w, int cr) { ki::set_lu_thresholds(w, cr); }
+      ~Thresholds() { ki::set_lu_thresholds(0, 0); }
+    } th(c.prm.lu_wide_rows, c.prm.lu_coop_rows);
     if (c.prec == 32)
       ki::skeletonize_t<float>(c);
     else

@@ -439,12 +439,29 @@ __device__ __forceinline__ void coop_barrier(unsigned* count, unsigned target,
 
 template <typename T>
 __global__ void __launch_bounds__(256)
-lu_coop_panel(T* __restrict__ A, int ld, int m, int j0, int jb,
-              int* __restrict__ piv, T* __restrict__ cv, int* __restrict__ ci,
-              T* __restrict__ crow, T* __restrict__ rowc,
-              unsigned* __restrict__ bar, int* __restrict__ fault) {
-  const int nwg = gridDim.x, b = blockIdx.x, tid = threadIdx.x;
+lu_coop_panel(const MatDesc* __restrict__ desc, T* __restrict__ a_all,
+              int* __restrict__ piv_all, int j0, int jend, int wg_max,
+              T* __restrict__ cv_all, int* __restrict__ ci_all,
+              T* __restrict__ crow_all, T* __restrict__ rowc_all,
+              unsigned* __restrict__ bar_all, int* __restrict__ fault) {
+  // matrix blockIdx.y of the batch; its panel is columns [j0, j0 + jb) with
+  // jb limited by min(m, n) and by jend (the enclosing outer panel's end)
+  const int box = blockIdx.y;
+  const MatDesc dm = desc[box];
+  const int lim = min(jend, min(dm.m, dm.n));
+  if (j0 >= lim) return;
+  const int jb = min(LB, lim - j0);
+  const int m = dm.m, ld = dm.ld;
+  T* A = a_all + dm.off;
+  int* piv = piv_all + dm.piv_off;
   const int mr = m - j0;
+  const int nwg = (mr + PR - 1) / PR, b = blockIdx.x, tid = threadIdx.x;
+  if (b >= nwg) return;  // this matrix needs fewer workgroups
+  T* cv = cv_all + (int64_t)box * LB * wg_max;
+  int* ci = ci_all + (int64_t)box * LB * wg_max;
+  T* crow = crow_all + (int64_t)box * LB * wg_max * LB;
+  T* rowc = rowc_all + (int64_t)box * LB * LB;
+  unsigned* bar = bar_all + box;
   const int r0 = b * PR;
   const int nr = min(PR, mr - r0);
   __shared__ T P[LB][PR + 1];  // P[q][i] = A[j0 + r0 + i, j0 + q]
@@ -834,6 +851,107 @@ static void trsm_sweeps(const std::vector<MatDesc>& h, MatDesc* d_desc,
 // panels (multi-workgroup column steps, interchanges and TRSM confined to the
 // outer panel, K = 32 GEMM inside it).  Same pivots as the one-level form.
 template <typename T>
+static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s);
+
+namespace {
+thread_local int g_lu_wide_rows = 0, g_lu_coop_rows = 0;
+// Set when the runtime refused a cooperative launch (for instance while
+// another process on the GPU holds the cooperative-launch resources): the
+// LUs of this process then use the per-column / one-workgroup panels.
+bool g_coop_refused = false;
+}  // namespace
+
+void set_lu_thresholds(int wide_rows, int coop_rows) {
+  g_lu_wide_rows = wide_rows;
+  g_lu_coop_rows = coop_rows;
+}
+
+// Scratch and launches of the cooperative panel kernel for a batch of
+// matrices that advance panel by panel together.
+template <typename T>
+struct CoopPanels {
+  int nb = 0, wg_max = 0, npanels = 0, panel_idx = 0;
+  MatDesc* d_desc = nullptr;
+  T *cv = nullptr, *crow = nullptr, *rowc = nullptr;
+  int* ci = nullptr;
+  unsigned* bar = nullptr;  // one arrival counter per (panel, matrix)
+  int* fault = nullptr;
+
+  // false (and nothing allocated) when the device cannot keep every
+  // workgroup of a panel launch resident, or KI_LU_NO_COOP is set
+  bool init(const std::vector<MatDesc>& h, hipStream_t s) {
+    if (std::getenv("KI_LU_NO_COOP") != nullptr || h.empty() ||
+        g_coop_refused)
+      return false;
+    nb = h.size();
+    int maxm = 0, maxmn = 0;
+    for (const MatDesc& x : h) {
+      maxm = std::max(maxm, x.m);
+      maxmn = std::max(maxmn, std::min(x.m, x.n));
+    }
+    wg_max = (maxm + PR - 1) / PR;
+    int dev = 0, ncu = 0, can = 0, per_cu = 0;
+    KI_HIP(hipGetDevice(&dev));
+    KI_HIP(hipDeviceGetAttribute(&can, hipDeviceAttributeCooperativeLaunch,
+                                 dev));
+    KI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                                 dev));
+    KI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(&lu_coop_panel<T>), 256, 0));
+    if (!can || (int64_t)per_cu * ncu < (int64_t)wg_max * nb) return false;
+    npanels = (maxmn + LB - 1) / LB;
+    const size_t slots = (size_t)LB * wg_max * nb;
+    KI_HIP(hipMallocAsync((void**)&d_desc, sizeof(MatDesc) * nb, s));
+    KI_HIP(hipMemcpyAsync(d_desc, h.data(), sizeof(MatDesc) * nb,
+                          hipMemcpyHostToDevice, s));
+    KI_HIP(hipMallocAsync((void**)&cv, sizeof(T) * slots, s));
+    KI_HIP(hipMallocAsync((void**)&ci, sizeof(int) * slots, s));
+    KI_HIP(hipMallocAsync((void**)&crow, sizeof(T) * slots * LB, s));
+    KI_HIP(hipMallocAsync((void**)&rowc, sizeof(T) * (size_t)LB * LB * nb,
+                          s));
+    const size_t nbar = (size_t)npanels * nb + 1;
+    KI_HIP(hipMallocAsync((void**)&bar, sizeof(unsigned) * nbar, s));
+    KI_HIP(hipMemsetAsync(bar, 0, sizeof(unsigned) * nbar, s));
+    fault = reinterpret_cast<int*>(bar + nbar - 1);
+    KI_HIP(hipStreamSynchronize(s));  // h may die after return
+    return true;
+  }
+
+  // Factor columns [j0, min(j0 + LB, jend)) of every matrix.  Returns false,
+  // with nothing launched, when the runtime refuses the cooperative launch;
+  // the caller then factors this and the remaining panels another way.
+  bool panel(T* a, int* piv_all, int j0, int jend, hipStream_t s) {
+    KI_CHECK(panel_idx < npanels, 2, "cooperative panel count exceeded");
+    unsigned* pbar = bar + (size_t)panel_idx * nb;
+    void* args[] = {&d_desc, &a,   &piv_all, &j0,   &jend, &wg_max,
+                    &cv,     &ci,  &crow,    &rowc, &pbar, &fault};
+    const hipError_t e = hipLaunchCooperativeKernel(
+        reinterpret_cast<const void*>(&lu_coop_panel<T>), dim3(wg_max, nb),
+        dim3(256), args, 0, s);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();  // clear the launch error
+      g_coop_refused = true;
+      return false;
+    }
+    panel_idx++;
+    return true;
+  }
+
+  void finish(hipStream_t s) {
+    int hf = 0;
+    KI_HIP(hipMemcpyAsync(&hf, fault, sizeof(int), hipMemcpyDeviceToHost, s));
+    KI_HIP(hipStreamSynchronize(s));
+    KI_HIP(hipFreeAsync(d_desc, s));
+    KI_HIP(hipFreeAsync(cv, s));
+    KI_HIP(hipFreeAsync(ci, s));
+    KI_HIP(hipFreeAsync(crow, s));
+    KI_HIP(hipFreeAsync(rowc, s));
+    KI_HIP(hipFreeAsync(bar, s));
+    KI_CHECK(hf == 0, 3, "LU panel grid barrier timed out");  // device error
+  }
+};
+
+template <typename T>
 static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
   constexpr int NB = 256;
   const int m = x.m, n = x.n, ld = x.ld, mn = std::min(m, n);
@@ -867,34 +985,9 @@ static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
                           hipMemcpyHostToDevice, s));
   KI_HIP(hipMallocAsync((void**)&d_m, sizeof(MatDesc), s));
   KI_HIP(hipMallocAsync((void**)&W, sizeof(T) * (size_t)m * LB, s));
-  // Cooperative panels need every workgroup of the grid resident at once.
-  const int coop_wg = (m + PR - 1) / PR;
-  bool coop = std::getenv("KI_LU_NO_COOP") == nullptr;
-  if (coop) {
-    int dev = 0, ncu = 0, per_cu = 0, can = 0;
-    KI_HIP(hipGetDevice(&dev));
-    KI_HIP(hipDeviceGetAttribute(&can, hipDeviceAttributeCooperativeLaunch,
-                                 dev));
-    KI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
-                                 dev));
-    KI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void*>(&lu_coop_panel<T>), 256, 0));
-    coop = can && (int64_t)per_cu * ncu >= coop_wg;
-  }
-  T *cv = nullptr, *crow = nullptr, *rowc = nullptr;
-  int* ci = nullptr;
-  unsigned* bar = nullptr;  // one arrival counter per panel, then the fault
-  const int npanels = (mn + LB - 1) / LB;
-  if (coop) {
-    KI_HIP(hipMallocAsync((void**)&cv, sizeof(T) * (size_t)LB * coop_wg, s));
-    KI_HIP(hipMallocAsync((void**)&ci, sizeof(int) * (size_t)LB * coop_wg, s));
-    KI_HIP(hipMallocAsync((void**)&crow,
-                          sizeof(T) * (size_t)LB * coop_wg * LB, s));
-    KI_HIP(hipMallocAsync((void**)&rowc, sizeof(T) * (size_t)LB * LB, s));
-    KI_HIP(hipMallocAsync((void**)&bar, sizeof(unsigned) * (npanels + 1), s));
-    KI_HIP(hipMemsetAsync(bar, 0, sizeof(unsigned) * (npanels + 1), s));
-  }
-  int* fault = coop ? reinterpret_cast<int*>(bar + npanels) : nullptr;
+  CoopPanels<T> cp;
+  const bool cp_live = cp.init(std::vector<MatDesc>{x}, s);
+  bool coop = cp_live;
   size_t gi = 0;
   auto cols = [](int c) { return dim3((std::max(c, 1) + 255) / 256); };
   for (int J0 = 0; J0 < mn; J0 += NB) {
@@ -903,15 +996,8 @@ static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
       const int jb = std::min(LB, J0 + JB - j0);
       const int mr = m - j0;
       const dim3 g((mr + 255) / 256);
-      if (coop) {
-        int a_ld = ld, a_m = m, a_j0 = j0, a_jb = jb;
-        unsigned* pbar = bar + j0 / LB;
-        void* args[] = {&A,  &a_ld, &a_m,  &a_j0, &a_jb, &piv,
-                        &cv, &ci,   &crow, &rowc, &pbar, &fault};
-        KI_HIP(hipLaunchCooperativeKernel(
-            reinterpret_cast<const void*>(&lu_coop_panel<T>),
-            dim3((mr + PR - 1) / PR), dim3(256), args, 0, s));
-      } else {
+      if (coop) coop = cp.panel(a, piv_all, j0, J0 + JB, s);
+      if (!coop) {
         for (int c = 0; c < jb; c++)
           hipLaunchKernelGGL(lu_pstep<T>, g, dim3(256), 0, s, A, ld, m, j0, jb,
                              W, c, piv, c & 1);
@@ -964,27 +1050,14 @@ static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
   KI_HIP(hipFreeAsync(d_g, s));
   KI_HIP(hipFreeAsync(d_m, s));
   KI_HIP(hipFreeAsync(W, s));
-  if (coop) {
-    KI_HIP(hipFreeAsync(cv, s));
-    KI_HIP(hipFreeAsync(ci, s));
-    KI_HIP(hipFreeAsync(crow, s));
-    KI_HIP(hipFreeAsync(rowc, s));
-    int hf = 0;
-    KI_HIP(hipMemcpyAsync(&hf, fault, sizeof(int), hipMemcpyDeviceToHost, s));
-    KI_HIP(hipStreamSynchronize(s));
-    KI_HIP(hipFreeAsync(bar, s));
-    KI_CHECK(hf == 0, 3, "LU panel grid barrier timed out");  // device error
-  }
+  if (cp_live) cp.finish(s);
 }
 
 template <typename T>
 void batched_lu(const std::vector<MatDesc>& h, MatDesc* d_desc, T* a, int* piv,
                 hipStream_t s) {
   if (h.empty()) return;
-  static const int wide_rows = [] {
-    const char* e = std::getenv("KI_WIDE_LU_ROWS");  // test override
-    return e ? std::atoi(e) : 4096;
-  }();
+  const int wide_rows = g_lu_wide_rows > 0 ? g_lu_wide_rows : 4096;
   if (h.size() == 1 && h[0].m > wide_rows) {
     lu_wide<T>(h[0], a, piv, s);
     return;
@@ -1027,10 +1100,20 @@ void batched_lu(const std::vector<MatDesc>& h, MatDesc* d_desc, T* a, int* piv,
   KI_HIP(hipMallocAsync((void**)&d_g, sizeof(GemmDesc) * gd.size(), s));
   KI_HIP(hipMemcpyAsync(d_g, gd.data(), sizeof(GemmDesc) * gd.size(),
                         hipMemcpyHostToDevice, s));
+  // Tall matrices: each panel spread over many workgroups per matrix (one
+  // cooperative launch for the whole batch); short ones: one workgroup each.
+  const int coop_rows = g_lu_coop_rows > 0 ? g_lu_coop_rows : 1024;
+  int maxm = 0;
+  for (auto& x : h) maxm = std::max(maxm, x.m);
+  CoopPanels<T> cp;
+  const bool cp_live = maxm > coop_rows && cp.init(h, s);
+  bool coop = cp_live;
   for (int p = 0; p < npan; p++) {
     const int j0 = p * LB;
-    hipLaunchKernelGGL(lu_panel<T>, dim3(nb), dim3(256), 0, s, d_desc, a, piv,
-                       j0);
+    if (coop) coop = cp.panel(a, piv, j0, INT_MAX, s);
+    if (!coop)
+      hipLaunchKernelGGL(lu_panel<T>, dim3(nb), dim3(256), 0, s, d_desc, a,
+                         piv, j0);
     hipLaunchKernelGGL(lu_swap_trsm<T>, dim3((maxn + 255) / 256, nb),
                        dim3(256), 0, s, d_desc, a, piv, j0);
     if (mm[p] > 0 && nn[p] > 0)
@@ -1038,6 +1121,7 @@ void batched_lu(const std::vector<MatDesc>& h, MatDesc* d_desc, T* a, int* piv,
                      0);
   }
   KI_HIP(hipGetLastError());
+  if (cp_live) cp.finish(s);
   // the host descriptor vector dies here: make sure the upload has landed
   KI_HIP(hipStreamSynchronize(s));
   KI_HIP(hipFreeAsync(d_g, s));

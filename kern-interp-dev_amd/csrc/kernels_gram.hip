@@ -405,22 +405,61 @@ __global__ void pchol_init(const CholDesc* __restrict__ desc,
   ws_all[d.d_off + j] = g_all[d.g_off + j + (int64_t)n * j];
 }
 
-// Finish the large boxes: rank, redundant order, R = L^T over G.
-__global__ void __launch_bounds__(256)
-pchol_finish(const CholDesc* __restrict__ desc, double* __restrict__ g_all,
-             double* __restrict__ ws_all, int* __restrict__ piv_all,
-             const int* __restrict__ state, int* __restrict__ rank_out,
-             int steps_run) {
+// Finish the large boxes, part 1: rank out, and the redundant columns (the
+// unselected ones, ascending) appended to the pivot order.  One thread per
+// box: a scan of n flags.
+__global__ void pchol_order(const CholDesc* __restrict__ desc,
+                            const double* __restrict__ ws_all,
+                            int* __restrict__ piv_all,
+                            const int* __restrict__ state,
+                            int* __restrict__ rank_out) {
   const CholDesc d = desc[blockIdx.x];
+  if (threadIdx.x != 0) return;
   const int n = d.n;
   const int rank = state[4 * d.slot];
-  if (threadIdx.x == 0) rank_out[d.slot] = rank;
+  rank_out[d.slot] = rank;
   if (rank == 0) return;
   // the live diagonal buffer is the one written by the last executed step
-  const int last = rank;  // steps 0..rank-1 ran; step `rank` only stopped
-  const double* dg = ws_all + d.d_off + ((last & 1) ? n : 0);
-  (void)steps_run;
-  write_r(g_all + d.g_off, ws_all + d.l_off, dg, piv_all + d.piv_off, n, rank);
+  // (steps 0..rank-1 ran; step `rank` only detected the stop)
+  const double* dg = ws_all + d.d_off + ((rank & 1) ? n : 0);
+  int* piv = piv_all + d.piv_off;
+  int q = rank;
+  for (int j = 0; j < n; j++)
+    if (dg[j] >= 0.0) piv[q++] = j;
+}
+
+// Part 2: R = L^T in pivot order over G (rank x n, ld n): R(a, c) =
+// L(piv[c], a), zero below the diagonal of the leading rank x rank block.
+// 64 x 64 tiles transposed through LDS: the gathers of L run along c, the
+// stores of R along a (grid: column tiles, row tiles, boxes).
+__global__ void __launch_bounds__(256)
+pchol_write_r(const CholDesc* __restrict__ desc, double* __restrict__ g_all,
+              const double* __restrict__ ws_all,
+              const int* __restrict__ piv_all, const int* __restrict__ state) {
+  const CholDesc d = desc[blockIdx.z];
+  const int n = d.n;
+  const int rank = state[4 * d.slot];
+  const int c0 = blockIdx.x * 64, a0 = blockIdx.y * 64;
+  if (rank == 0 || c0 >= n || a0 >= rank) return;
+  const double* L = ws_all + d.l_off;
+  const int* piv = piv_all + d.piv_off;
+  double* G = g_all + d.g_off;
+  __shared__ double tile[64][65];  // tile[cc][aa] = L(piv[c0 + cc], a0 + aa)
+  const int lo = threadIdx.x & 63, hi = threadIdx.x >> 6;
+  const int c = c0 + lo;
+  const int pc = c < n ? piv[c] : 0;
+  for (int aa = hi; aa < 64; aa += 4) {
+    const int a = a0 + aa;
+    tile[lo][aa] = (c < n && a < rank) ? L[pc + (int64_t)n * a] : 0.0;
+  }
+  __syncthreads();
+  const int a = a0 + lo;
+  if (a >= rank) return;
+  for (int cc = hi; cc < 64; cc += 4) {
+    const int cw = c0 + cc;
+    if (cw >= n) break;
+    G[a + (int64_t)n * cw] = (a <= cw || cw >= rank) ? tile[cc][lo] : 0.0;
+  }
 }
 
 }  // namespace
@@ -506,8 +545,13 @@ void pchol_large(const std::vector<CholDesc>& h, const CholDesc* d_desc,
       if (all) break;
     }
   }
-  hipLaunchKernelGGL(pchol_finish, dim3(nb), dim3(256), 0, s, d_desc, g, ws,
-                     piv, state, rank, i);
+  (void)i;
+  hipLaunchKernelGGL(pchol_order, dim3(nb), dim3(64), 0, s, d_desc,
+                     (const double*)ws, piv, (const int*)state, rank);
+  const int t64 = (maxn + 63) / 64;
+  hipLaunchKernelGGL(pchol_write_r, dim3(t64, t64, nb), dim3(256), 0, s,
+                     d_desc, g, (const double*)ws, (const int*)piv,
+                     (const int*)state);
   KI_HIP(hipGetLastError());
   KI_HIP(hipStreamSynchronize(s));  // descriptor vector dies with this scope
   KI_HIP(hipFreeAsync(d_g, s));

@@ -226,64 +226,81 @@ __device__ __forceinline__ double rsqrt_nr(double x) {
 
 // Targets are the boundary points themselves (self = 1, the residual
 // operator) or arbitrary domain points with zero normal (self = 0, the
-// forward evaluation of Kernel::apply_forward).
+// forward evaluation of Kernel::apply_forward).  Specialized on the domain
+// dimension DOM and unknowns per point SOL so the pair loop is straight-line
+// code; the self pair (r = 0) is masked by giving it zero weight.
+template <int DOM, int SOL>
 __global__ void __launch_bounds__(MT)
 matvec_kernel(DevGeom g, const double* __restrict__ tpts,
               const double* __restrict__ tnrm, int ntgt, int self,
               const double* __restrict__ mu, double* __restrict__ y) {
   const double PI = 3.14159265358979323846;
-  const int dom = g.dom, sol = g.sol, n = g.npts;
+  const int dom = DOM, sol = SOL, n = g.npts;
   const int i = blockIdx.x * MT + threadIdx.x;
-  __shared__ double sx[MT][3], sn[MT][3], sw[MT], sm[MT][3];
-  double tx[3] = {0, 0, 0}, tn[3] = {0, 0, 0};
-  if (i < ntgt)
-    for (int a = 0; a < dom; a++) {
-      tx[a] = tpts[(int64_t)i * dom + a];
-      tn[a] = tnrm ? tnrm[(int64_t)i * dom + a] : 0.0;
-    }
-  double acc[3] = {0, 0, 0};
+  __shared__ double sx[MT][DOM], sn[MT][DOM], sw[MT], sm[MT][SOL];
+  double tx[DOM], tn[DOM];
+#pragma unroll
+  for (int a = 0; a < DOM; a++) {
+    tx[a] = i < ntgt ? tpts[(int64_t)i * DOM + a] : 0.0;
+    tn[a] = (i < ntgt && tnrm) ? tnrm[(int64_t)i * DOM + a] : 0.0;
+  }
+  double acc[SOL];
+#pragma unroll
+  for (int a = 0; a < SOL; a++) acc[a] = 0.0;
+  const int me = self && i < ntgt ? i : -1;
   for (int j0 = 0; j0 < n; j0 += MT) {
     const int j = j0 + threadIdx.x;
     __syncthreads();
     if (j < n) {
-      for (int a = 0; a < 3; a++) {
-        sx[threadIdx.x][a] = a < dom ? g.pts[(int64_t)j * dom + a] : 0.0;
-        sn[threadIdx.x][a] = a < dom ? g.nrm[(int64_t)j * dom + a] : 0.0;
-        sm[threadIdx.x][a] = a < sol ? mu[(int64_t)j * sol + a] : 0.0;
+#pragma unroll
+      for (int a = 0; a < DOM; a++) {
+        sx[threadIdx.x][a] = g.pts[(int64_t)j * DOM + a];
+        sn[threadIdx.x][a] = g.nrm[(int64_t)j * DOM + a];
       }
+#pragma unroll
+      for (int a = 0; a < SOL; a++)
+        sm[threadIdx.x][a] = mu[(int64_t)j * SOL + a];
       sw[threadIdx.x] = g.w[j];
     }
     __syncthreads();
     const int cnt = min(MT, n - j0);
-    if (i >= ntgt) continue;
+#pragma unroll 4
     for (int q = 0; q < cnt; q++) {
-      if (self && j0 + q == i) continue;
-      const double r0 = tx[0] - sx[q][0], r1 = tx[1] - sx[q][1],
-                   r2 = tx[2] - sx[q][2];
-      const double rr = r0 * r0 + r1 * r1 + r2 * r2;
-      const double dot = r0 * sn[q][0] + r1 * sn[q][1] + r2 * sn[q][2];
-      const double w = sw[q];
-      if (sol == 1) {
-        if (dom == 2)
+      double r[DOM], rr = 0.0, dot = 0.0;
+#pragma unroll
+      for (int a = 0; a < DOM; a++) {
+        r[a] = tx[a] - sx[q][a];
+        rr += r[a] * r[a];
+        dot += r[a] * sn[q][a];
+      }
+      const bool is_self = j0 + q == me;
+      const double w = is_self ? 0.0 : sw[q];
+      rr = is_self ? 1.0 : rr;
+      if (SOL == 1) {
+        if (DOM == 2) {
           acc[0] += -w * (1.0 / (2 * PI)) * dot / rr * sm[q][0];
-        else {
+        } else {
           const double ir = rsqrt_nr(rr);
           acc[0] += -w * (1.0 / (4 * PI)) * dot * (ir * ir * ir) * sm[q][0];
         }
       } else {
-        const double rm = r0 * sm[q][0] + r1 * sm[q][1] + r2 * sm[q][2];
-        const double nm = sn[q][0] * sm[q][0] + sn[q][1] * sm[q][1] +
-                          sn[q][2] * sm[q][2];
+        double rm = 0.0, nm = 0.0;
+#pragma unroll
+        for (int a = 0; a < DOM; a++) {
+          rm += r[a] * sm[q][a];
+          nm += sn[q][a] * sm[q][a];
+        }
         double f;
-        if (dom == 2) {
+        if (DOM == 2) {
           f = w * (1 / PI) * dot / (rr * rr);
         } else {
           const double ir = rsqrt_nr(rr);
           const double ir2 = ir * ir;
           f = w * (-3.0 / (4.0 * PI)) * dot * (ir2 * ir2 * ir);
         }
-        const double rv[3] = {r0, r1, r2};
-        for (int a = 0; a < sol; a++) acc[a] += f * rv[a] * rm + w * tn[a] * nm;
+        const double wnm = w * nm;
+#pragma unroll
+        for (int a = 0; a < SOL; a++) acc[a] += f * r[a] * rm + tn[a] * wnm;
       }
     }
   }
@@ -294,14 +311,14 @@ matvec_kernel(DevGeom g, const double* __restrict__ tpts,
   }
   // self term
   const double w = g.w[i];
-  if (sol == 1) {
+  if constexpr (SOL == 1) {
     double dg;
-    if (dom == 2)
+    if (DOM == 2)
       dg = 0.5 + 0.5 * g.curv[i] * w * (1.0 / (2 * PI));
     else
       dg = g.diag[i];
     y[i] = acc[0] + dg * mu[i];
-  } else if (dom == 2) {
+  } else if constexpr (DOM == 2) {
     const double c = g.curv[i];
     for (int a = 0; a < 2; a++) {
       double s = acc[a];
@@ -463,19 +480,36 @@ template void launch_diag_blocked<float, double>(const DiagDesc*, int, int,
                                                  int64_t, double* const*,
                                                  float*, double*, hipStream_t);
 
+static void matvec_any(const DevGeom& g, const double* tpts,
+                       const double* tnrm, int ntgt, int self,
+                       const double* mu, double* y, hipStream_t s) {
+  const dim3 grid((ntgt + MT - 1) / MT), blk(MT);
+  if (g.dom == 2 && g.sol == 1)
+    hipLaunchKernelGGL((matvec_kernel<2, 1>), grid, blk, 0, s, g, tpts, tnrm,
+                       ntgt, self, mu, y);
+  else if (g.dom == 2 && g.sol == 2)
+    hipLaunchKernelGGL((matvec_kernel<2, 2>), grid, blk, 0, s, g, tpts, tnrm,
+                       ntgt, self, mu, y);
+  else if (g.dom == 3 && g.sol == 1)
+    hipLaunchKernelGGL((matvec_kernel<3, 1>), grid, blk, 0, s, g, tpts, tnrm,
+                       ntgt, self, mu, y);
+  else if (g.dom == 3 && g.sol == 3)
+    hipLaunchKernelGGL((matvec_kernel<3, 3>), grid, blk, 0, s, g, tpts, tnrm,
+                       ntgt, self, mu, y);
+  else
+    fail(2, "unsupported dimension / PDE for the matvec");
+  KI_HIP(hipGetLastError());
+}
+
 void launch_matvec(const DevGeom& g, const double* mu, double* y,
                    hipStream_t s) {
-  hipLaunchKernelGGL(matvec_kernel, dim3((g.npts + MT - 1) / MT), dim3(MT), 0,
-                     s, g, g.pts, g.nrm, g.npts, 1, mu, y);
-  KI_HIP(hipGetLastError());
+  matvec_any(g, g.pts, g.nrm, g.npts, 1, mu, y, s);
 }
 
 void launch_forward(const DevGeom& g, const double* tpts, int ntgt,
                     const double* mu, double* y, hipStream_t s) {
   if (ntgt <= 0) return;
-  hipLaunchKernelGGL(matvec_kernel, dim3((ntgt + MT - 1) / MT), dim3(MT), 0,
-                     s, g, tpts, nullptr, ntgt, 0, mu, y);
-  KI_HIP(hipGetLastError());
+  matvec_any(g, tpts, nullptr, ntgt, 0, mu, y, s);
 }
 
 void launch_diag3d(DevGeom& g, hipStream_t s) {

@@ -233,6 +233,12 @@ template <typename T>
 void batched_lu(const std::vector<MatDesc>& h, MatDesc* d_desc, T* a,
                 int* piv, hipStream_t s);
 
+// Row thresholds of the LU forms for the calling thread (0 keeps the
+// default): a single matrix taller than wide_rows takes the two-level
+// blocked form; batches taller than coop_rows factor each panel over many
+// workgroups.
+void set_lu_thresholds(int wide_rows, int coop_rows);
+
 // X = U^{-1} B for upper-triangular U (no interchanges), blocked.
 template <typename T>
 void batched_trsm_upper(const std::vector<MatDesc>& h, MatDesc* d_desc,
