@@ -8,6 +8,8 @@
 // (ki_mat.cpp:328-335, 445-513) inside decouple (skel_factorization.cpp:
 // 91-127), the top-level LU (:285-310) and the block-diagonal solves of the
 // sweep (:585-592, 676-722).
+#include <rocblas/rocblas.h>
+
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -853,6 +855,36 @@ static void trsm_sweeps(const std::vector<MatDesc>& h, MatDesc* d_desc,
 template <typename T>
 static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s);
 
+// C = alpha op(A) op(B) + beta C for one descriptor through rocBLAS (one
+// handle per host thread, bound to the caller's stream).  Used only for the
+// plain, large, unbatched products of the top-level LU; false when rocBLAS
+// is unavailable or declines, in which case the caller launches its own.
+template <typename T>
+static bool library_gemm(const GemmDesc& q, T* base, hipStream_t s) {
+  if (std::getenv("KI_NO_ROCBLAS") != nullptr) return false;
+  thread_local rocblas_handle h = nullptr;
+  thread_local bool bad = false;
+  if (bad) return false;
+  if (!h && rocblas_create_handle(&h) != rocblas_status_success) {
+    bad = true;
+    return false;
+  }
+  if (rocblas_set_stream(h, s) != rocblas_status_success) return false;
+  const rocblas_operation ta =
+      q.ta ? rocblas_operation_transpose : rocblas_operation_none;
+  const rocblas_operation tb =
+      q.tb ? rocblas_operation_transpose : rocblas_operation_none;
+  const T alpha = (T)q.alpha, beta = (T)q.beta;
+  rocblas_status st;
+  if constexpr (sizeof(T) == 8)
+    st = rocblas_dgemm(h, ta, tb, q.m, q.n, q.k, &alpha, base + q.a_off, q.lda,
+                       base + q.b_off, q.ldb, &beta, base + q.c_off, q.ldc);
+  else
+    st = rocblas_sgemm(h, ta, tb, q.m, q.n, q.k, &alpha, base + q.a_off, q.lda,
+                       base + q.b_off, q.ldb, &beta, base + q.c_off, q.ldc);
+  return st == rocblas_status_success;
+}
+
 namespace {
 thread_local int g_lu_wide_rows = 0, g_lu_coop_rows = 0;
 // Set when the runtime refused a cooperative launch (for instance while
@@ -1039,8 +1071,11 @@ static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
       const int mode0[1] = {0};
       trsm_sweeps<T>(hv, d_m, a, nullptr, a, mode0, 1, 0, s);
       if (m > J0 + JB) {
+        // the K = 256 trailing update is a plain GEMM: the library's
+        // tuned kernel when it is available, else the batched one here
         const GemmDesc& q = gd[gi];
-        launch_gemm<T>(d_g + gi, 1, q.m, q.n, a, a, a, s, 0, 0);
+        if (!library_gemm<T>(q, a, s))
+          launch_gemm<T>(d_g + gi, 1, q.m, q.n, a, a, a, s, 0, 0);
         gi++;
       }
     }
