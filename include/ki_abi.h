@@ -60,7 +60,9 @@ typedef struct ki_params {
                                blocked form (0: library default, 4096)        */
   int lu_coop_rows;         /* batched LUs taller than this factor each panel
                                over many workgroups (0: default, 1024)        */
-  int reserved[6];
+  int solve_graph;          /* 1: capture the factor apply once per
+                               factorization in a HIP graph and replay it    */
+  int reserved[5];
 } ki_params;
 
 void ki_params_default(ki_params* p);
@@ -167,6 +169,13 @@ int ki_lu_solve(ki_ctx* ctx, const double* lu, const int* piv, int n,
 int ki_gemm(ki_ctx* ctx, int precision, int ta, int tb, int m, int n, int k,
             double alpha, const double* a, int lda, const double* b, int ldb,
             double beta, double* c, int ldc);
+
+/* Decisions of the batched-GEMM tile autotuner in this process: *count
+ * classes, 8 ints each {element bytes, ta, tb, ceil log2 max m, ceil log2
+ * max n, ceil log2 batch count, tile (0: 64 x 64, 1: 128 x 128, -1: not yet
+ * decided), 1000 * time(64) / time(128)}; copies min(cap, count) classes.
+ * No reference counterpart. */
+int ki_gemm_tuning(int* out, int cap, int* count);
 
 /* Factorization statistics. */
 typedef struct ki_stats {

@@ -34,7 +34,7 @@ EXPORTS = [
     "ki_apply_forward", "ki_get_stats", "ki_box_log", "ki_proxy_block",
     "ki_lu_factorize", "ki_lu_solve", "ki_kernel_clock", "ki_box_features",
     "ki_mlp_set_params", "ki_mlp_get_params", "ki_mlp_init", "ki_mlp_grad",
-    "ki_mlp_train", "ki_mlp_predict", "ki_gemm",
+    "ki_mlp_train", "ki_mlp_predict", "ki_gemm", "ki_gemm_tuning",
 ]
 
 MLP_NFEAT, MLP_NPARAM = 12, 5057
@@ -54,7 +54,8 @@ class KiParams(C.Structure):
         ("fp32_tol_floor", C.c_double),
         ("lu_wide_rows", C.c_int),
         ("lu_coop_rows", C.c_int),
-        ("reserved", C.c_int * 6),
+        ("solve_graph", C.c_int),
+        ("reserved", C.c_int * 5),
     ]
 
 
@@ -143,6 +144,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "ki_mlp_train": (C.c_int, [vp, fp, fp, C.c_int, C.c_int, C.c_int,
                                    C.c_float, C.c_uint64, fp]),
         "ki_mlp_predict": (C.c_int, [vp, fp, C.c_int, C.c_int, fp]),
+        "ki_gemm_tuning": (C.c_int, [ip, C.c_int, ip]),
         "ki_gemm": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                               C.c_int, C.c_double, dp, C.c_int, dp, C.c_int,
                               C.c_double, dp, C.c_int]),
@@ -180,6 +182,18 @@ def _fptr(a: np.ndarray):
 def _f32(a, shape=None) -> np.ndarray:
     a = np.ascontiguousarray(a, dtype=np.float32)
     return a if shape is None else a.reshape(shape)
+
+
+def gemm_tuning() -> list:
+    """The batched-GEMM tile autotuner's decisions in this process."""
+    lib = load_library()
+    cnt = C.c_int()
+    _check(lib.ki_gemm_tuning(None, 0, C.byref(cnt)))
+    out = np.zeros((max(cnt.value, 1), 8), dtype=np.int32)
+    _check(lib.ki_gemm_tuning(_iptr(out), cnt.value, C.byref(cnt)))
+    keys = ("elem_bytes", "ta", "tb", "log2_m", "log2_n", "log2_batch",
+            "tile128", "t64_over_t128_x1000")
+    return [dict(zip(keys, map(int, r))) for r in out[:cnt.value]]
 
 
 def default_params(**kw) -> KiParams:

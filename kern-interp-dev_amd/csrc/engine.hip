@@ -149,7 +149,15 @@ struct Ctx {
       }
     kclk.push_back(KClock{name, launches, sec, flops, bytes});
   }
+  // the factor apply captured as a HIP graph over fixed staging vectors
+  hipGraphExec_t solve_exec = nullptr;
+  DevBuf g_in, g_out;
+  void drop_solve_graph() {
+    if (solve_exec) (void)hipGraphExecDestroy(solve_exec);
+    solve_exec = nullptr;
+  }
   ~Ctx() {
+    drop_solve_graph();
     if (s) (void)hipStreamDestroy(s);
   }
 };
@@ -344,6 +352,7 @@ void skeletonize_t(Ctx& c) {
     b.acc_n = 0;
     b.fac_level = -1;
   }
+  c.drop_solve_graph();  // it refers to the previous factorization's buffers
   c.lev.clear();
   for (int l = 0; l < L; l++) c.lev.emplace_back(new LevelDev());
   c.boxlog.clear();
@@ -1222,11 +1231,10 @@ void skeletonize_t(Ctx& c) {
 
 // x_out = A^{-1} b_in with the factorization (device FP64 vectors).
 template <typename T>
-void solve_t(Ctx& c, const double* b_in, double* x_out) {
+void solve_launches(Ctx& c, const double* b_in, double* x_out) {
   hipStream_t s = c.s;
   Tree& tr = c.tree;
   const int L = tr.levels.size();
-  c.d_xt.ensure(sizeof(T) * c.ndof);
   T* xt = c.d_xt.as<T>();
   launch_convert<T, double>(b_in, xt, c.ndof, s);
   T* const* fb = c.facptrs.as<T*>();
@@ -1263,6 +1271,42 @@ void solve_t(Ctx& c, const double* b_in, double* x_out) {
     phase(LD, 3);
   }
   launch_convert<double, T>(xt, x_out, c.ndof, s);
+}
+
+// x_out = A^{-1} b_in.  With ki_params.solve_graph the launch sequence is
+// captured once per factorization into a HIP graph over the staging vectors
+// g_in / g_out (kernel nodes only) and replayed; the copies in and out are
+// ordinary stream operations around the replay.
+template <typename T>
+void solve_t(Ctx& c, const double* b_in, double* x_out) {
+  c.d_xt.ensure(sizeof(T) * c.ndof);
+  if (!c.prm.solve_graph) {
+    solve_launches<T>(c, b_in, x_out);
+    return;
+  }
+  const size_t vb = sizeof(double) * c.ndof;
+  if (!c.solve_exec) {
+    c.g_in.ensure(vb);
+    c.g_out.ensure(vb);
+    KI_HIP(hipStreamSynchronize(c.s));
+    KI_HIP(hipStreamBeginCapture(c.s, hipStreamCaptureModeThreadLocal));
+    hipGraph_t g = nullptr;
+    try {
+      solve_launches<T>(c, c.g_in.as<double>(), c.g_out.as<double>());
+    } catch (...) {
+      (void)hipStreamEndCapture(c.s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    KI_HIP(hipStreamEndCapture(c.s, &g));
+    const hipError_t e =
+        hipGraphInstantiate(&c.solve_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    KI_HIP(e);
+  }
+  KI_HIP(hipMemcpyAsync(c.g_in.p, b_in, vb, hipMemcpyDeviceToDevice, c.s));
+  KI_HIP(hipGraphLaunch(c.solve_exec, c.s));
+  KI_HIP(hipMemcpyAsync(x_out, c.g_out.p, vb, hipMemcpyDeviceToDevice, c.s));
 }
 
 void solve_dev(Ctx& c, const double* b, double* x) {
@@ -1451,6 +1495,7 @@ void ki_params_default(ki_params* p) {
   p->use_predictor = 0;
   p->id_method = 0;
   p->fp32_tol_floor = 1e-6;
+  p->solve_graph = 1;
 }
 
 const char* ki_last_error(void) { return ki::g_err.c_str(); }
@@ -1937,6 +1982,16 @@ int ki_gemm(ki_ctx* ctx, int precision, int ta, int tb, int m, int n, int k,
     else
       gemm_host<double>(ctx->c, ta ? 1 : 0, tb ? 1 : 0, m, n, k, alpha, a,
                         lda, b, ldb, beta, c, ldc);
+  });
+}
+
+int ki_gemm_tuning(int* out, int cap, int* count) {
+  return guard([&] {
+    const auto t = ki::gemm_tuning_table();
+    if (count) *count = t.size();
+    const int nc = std::max(0, std::min(cap, (int)t.size()));
+    for (int i = 0; out && i < nc; i++)
+      for (int j = 0; j < 8; j++) out[8 * i + j] = t[i][j];
   });
 }
 

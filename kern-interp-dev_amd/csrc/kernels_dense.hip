@@ -11,9 +11,12 @@
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
+#include <array>
 #include <climits>
 #include <cmath>
 #include <cstdlib>
+#include <map>
+#include <mutex>
 
 #include "ki_internal.h"
 #include "mfma.h"
@@ -814,11 +817,72 @@ scatter_kernel(const ScatterDesc* __restrict__ desc, const TS* __restrict__ src_
 
 }  // namespace
 
+// Tile-shape autotuner for the batched GEMM.  A product class is (element
+// size, ta, tb, ceil log2 of max m, max n and the batch count).  Its first
+// two launches run the 64 x 64 and the 128 x 128 kernel on the real data
+// (both compute the same product) under HIP events; every later launch of
+// the class uses the faster one.  Classes narrower than 256 rows or columns
+// use the 64 x 64 kernel untimed.  KI_GEMM_TILE = 64 / 128 forces a shape.
+struct TileChoice {
+  float ms[2] = {-1.f, -1.f};
+  int best = -1;
+};
+
+static int ceil_log2(int x) {
+  int l = 0;
+  while ((1 << l) < x) l++;
+  return l;
+}
+
+static std::mutex g_tune_mu;
+static std::map<uint64_t, TileChoice> g_tune;
+
+std::vector<std::array<int, 8>> gemm_tuning_table() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  std::vector<std::array<int, 8>> out;
+  for (const auto& kv : g_tune) {
+    const uint64_t k = kv.first;
+    out.push_back({(int)(k >> 40 & 0xff), (int)(k >> 32 & 0xff),
+                   (int)(k >> 24 & 0xff), (int)(k >> 16 & 0xff),
+                   (int)(k >> 8 & 0xff), (int)(k & 0xff), kv.second.best,
+                   (int)(1000 * (kv.second.ms[1] > 0 && kv.second.ms[0] > 0
+                                     ? kv.second.ms[0] / kv.second.ms[1]
+                                     : 0))});
+  }
+  return out;
+}
+
 template <typename T>
 void launch_gemm(const GemmDesc* d_desc, int ndesc, int max_m, int max_n,
                  const T* a, const T* b, T* c, hipStream_t s, int ta, int tb) {
   if (ndesc == 0 || max_m == 0 || max_n == 0) return;
-  const bool big = max_m >= 512 && max_n >= 512;
+  bool big = false;
+  int tuning = -1;  // variant being timed by this launch
+  TileChoice* tc = nullptr;
+  const char* force = std::getenv("KI_GEMM_TILE");
+  if (force) {
+    big = std::atoi(force) == 128;
+  } else if (max_m >= 256 && max_n >= 256) {
+    const uint64_t key = (uint64_t)sizeof(T) << 40 | (uint64_t)ta << 32 |
+                         (uint64_t)tb << 24 |
+                         (uint64_t)ceil_log2(max_m) << 16 |
+                         (uint64_t)ceil_log2(max_n) << 8 |
+                         (uint64_t)ceil_log2(ndesc);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tc = &g_tune[key];
+    if (tc->best >= 0) {
+      big = tc->best == 1;
+    } else {
+      tuning = tc->ms[0] < 0 ? 0 : 1;
+      big = tuning == 1;
+    }
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (tuning >= 0) {
+    KI_HIP(hipEventCreate(&e0));
+    KI_HIP(hipEventCreate(&e1));
+    KI_HIP(hipEventRecord(e0, s));
+  }
   const int tile = big ? H : GT;
   const int tiles = ((max_m + tile - 1) / tile) * ((max_n + tile - 1) / tile);
   for (int base = 0; base < ndesc; base += 65535) {
@@ -841,6 +905,18 @@ void launch_gemm(const GemmDesc* d_desc, int ndesc, int max_m, int max_n,
                          b, c);
   }
   KI_HIP(hipGetLastError());
+  if (tuning >= 0) {
+    KI_HIP(hipEventRecord(e1, s));
+    KI_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    KI_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tc->ms[tuning] = ms;
+    if (tc->ms[0] >= 0 && tc->ms[1] >= 0)
+      tc->best = tc->ms[1] < tc->ms[0] ? 1 : 0;
+  }
 }
 
 template <typename T>
@@ -855,12 +931,9 @@ static void trsm_sweeps(const std::vector<MatDesc>& h, MatDesc* d_desc,
 template <typename T>
 static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s);
 
-// C = alpha op(A) op(B) + beta C for one descriptor through rocBLAS (one
-// handle per host thread, bound to the caller's stream).  Used only for the
-// plain, large, unbatched products of the top-level LU; false when rocBLAS
-// is unavailable or declines, in which case the caller launches its own.
 template <typename T>
-static bool library_gemm(const GemmDesc& q, T* base, hipStream_t s) {
+bool library_gemm(const GemmDesc& q, const T* a, const T* b, T* c,
+                  hipStream_t s) {
   if (std::getenv("KI_NO_ROCBLAS") != nullptr) return false;
   thread_local rocblas_handle h = nullptr;
   thread_local bool bad = false;
@@ -877,13 +950,17 @@ static bool library_gemm(const GemmDesc& q, T* base, hipStream_t s) {
   const T alpha = (T)q.alpha, beta = (T)q.beta;
   rocblas_status st;
   if constexpr (sizeof(T) == 8)
-    st = rocblas_dgemm(h, ta, tb, q.m, q.n, q.k, &alpha, base + q.a_off, q.lda,
-                       base + q.b_off, q.ldb, &beta, base + q.c_off, q.ldc);
+    st = rocblas_dgemm(h, ta, tb, q.m, q.n, q.k, &alpha, a + q.a_off, q.lda,
+                       b + q.b_off, q.ldb, &beta, c + q.c_off, q.ldc);
   else
-    st = rocblas_sgemm(h, ta, tb, q.m, q.n, q.k, &alpha, base + q.a_off, q.lda,
-                       base + q.b_off, q.ldb, &beta, base + q.c_off, q.ldc);
+    st = rocblas_sgemm(h, ta, tb, q.m, q.n, q.k, &alpha, a + q.a_off, q.lda,
+                       b + q.b_off, q.ldb, &beta, c + q.c_off, q.ldc);
   return st == rocblas_status_success;
 }
+template bool library_gemm<float>(const GemmDesc&, const float*, const float*,
+                                  float*, hipStream_t);
+template bool library_gemm<double>(const GemmDesc&, const double*,
+                                   const double*, double*, hipStream_t);
 
 namespace {
 thread_local int g_lu_wide_rows = 0, g_lu_coop_rows = 0;
@@ -1074,7 +1151,7 @@ static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
         // the K = 256 trailing update is a plain GEMM: the library's
         // tuned kernel when it is available, else the batched one here
         const GemmDesc& q = gd[gi];
-        if (!library_gemm<T>(q, a, s))
+        if (!library_gemm<T>(q, a, a, a, s))
           launch_gemm<T>(d_g + gi, 1, q.m, q.n, a, a, a, s, 0, 0);
         gi++;
       }

@@ -533,9 +533,24 @@ void pchol_large(const std::vector<CholDesc>& h, const CholDesc* d_desc,
     const int i0 = i - i % CB;
     hipLaunchKernelGGL(pchol_step, grid, dim3(256), 0, s, d_desc, g, ws, piv,
                        state, i, i0, i & 1, tol);
-    if (i % CB == CB - 1 && i + 1 < maxsteps)
-      launch_gemm<double>(d_g + (size_t)(i / CB) * nb, nb, maxn, maxn, ws, ws,
-                          g, s, 0, 1);
+    if (i % CB == CB - 1 && i + 1 < maxsteps) {
+      // S -= L_blk L_blk^T is a plain GEMM per box: for a few large boxes
+      // the library's tuned kernel, else one batched launch
+      const size_t base = (size_t)(i / CB) * nb;
+      bool lib = nb <= 128 && maxn >= 512, any = false;
+      for (int b = 0; lib && b < nb; b++) {
+        const GemmDesc& q = gd[base + b];
+        if (q.m == 0) continue;
+        if (library_gemm<double>(q, ws, ws, g, s)) {
+          any = true;
+        } else {
+          KI_CHECK(!any, 3, "rocBLAS GEMM failed after earlier successes");
+          lib = false;
+        }
+      }
+      if (!lib)
+        launch_gemm<double>(d_g + base, nb, maxn, maxn, ws, ws, g, s, 0, 1);
+    }
     if ((i & 63) == 63) {  // stop early once every box has met its rank
       KI_HIP(hipMemcpyAsync(hst.data(), state, sizeof(int) * 4 * nb,
                             hipMemcpyDeviceToHost, s));

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <array>
 #include <vector>
 
 namespace ki {
@@ -212,8 +213,21 @@ struct GemmDesc {
   double alpha, beta;
 };
 
+// One plain (unbatched) GEMM through rocBLAS, one handle per host thread on
+// the caller's stream; offsets of q are applied to a, b, c.  Returns false
+// when the library is unavailable or declines (the caller then launches
+// the batched kernel instead).  KI_NO_ROCBLAS switches it off.
+template <typename T>
+bool library_gemm(const GemmDesc& q, const T* a, const T* b, T* c,
+                  hipStream_t s);
+
+// The GEMM tile autotuner's decisions so far: per product class {element
+// bytes, ta, tb, log2 max m, log2 max n, log2 batch, chosen tile (0: 64,
+// 1: 128, -1: undecided), 1000 * t64 / t128}.
+std::vector<std::array<int, 8>> gemm_tuning_table();
+
 // ta / tb: the (common) transposition flags of every descriptor in the
-// launch; they select the 128 x 128-tile kernel for large products.
+// launch; together with the sizes they key the tile-shape autotuner.
 template <typename T>
 void launch_gemm(const GemmDesc* d_desc, int ndesc, int max_m, int max_n,
                  const T* a, const T* b, T* c, hipStream_t s, int ta, int tb);

@@ -360,3 +360,46 @@ def test_gemm_matches_numpy(m, n, k, ta, tb, precision):
     want = -0.75 * (a.T if ta else a) @ (b.T if tb else b) + 1.25 * c
     tol = 1e-13 if precision == 64 else 1e-5
     assert np.linalg.norm(got - want) <= tol * np.linalg.norm(want)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_solve_graph_replays_match_eager_solves(precision):
+    """The factor apply captured in a HIP graph and replayed (with eager
+    GPU work and synchronizations between the replays) gives the eager
+    result on every replay: bit-equal in FP64, 1e-6 relative in FP32."""
+    b, pde, kw, leaf, prm, ctx, okern, otree = setup("stokes3d_ellipsoid")
+    rng = np.random.default_rng(21)
+    rhs = [rng.normal(size=ctx.ndof) for _ in range(4)]
+    ctx.skeletonize(ki.default_params(precision=precision, solve_graph=0,
+                                      **kw))
+    eager = [ctx.solve(r) for r in rhs]
+    ctx.skeletonize(ki.default_params(precision=precision, solve_graph=1,
+                                      **kw))
+    for rep in range(3):
+        for r, x0 in zip(rhs, eager):
+            x = ctx.solve(r)
+            ctx.matvec(x)             # eager work between replays
+            tol = 0.0 if precision == 64 else 1e-6
+            assert np.linalg.norm(x - x0) <= tol * np.linalg.norm(x0), rep
+
+
+def test_gemm_tile_shapes_and_autotuner(monkeypatch):
+    """Both tile shapes are correct on a large product, and the autotuner
+    records a decision for the class after two timed launches."""
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((900, 300))
+    b = rng.standard_normal((300, 700))
+    want = a @ b
+    ctx = ki.Context()
+    for tile in ("64", "128"):
+        monkeypatch.setenv("KI_GEMM_TILE", tile)
+        got = ctx.gemm(a, b)
+        assert np.linalg.norm(got - want) <= 1e-13 * np.linalg.norm(want)
+    monkeypatch.delenv("KI_GEMM_TILE")
+    for _ in range(3):
+        got = ctx.gemm(a, b)
+        assert np.linalg.norm(got - want) <= 1e-13 * np.linalg.norm(want)
+    table = ki.gemm_tuning()
+    mine = [t for t in table if t["elem_bytes"] == 8 and t["log2_m"] == 10
+            and t["log2_n"] == 10 and t["log2_batch"] == 0]
+    assert mine and mine[0]["tile128"] in (0, 1)
