@@ -130,11 +130,65 @@ gemm_kernel(const GemmDesc* __restrict__ desc, const T* __restrict__ a_all,
 // 64 x 64 (4 x 4 MFMA tiles), K slices of 16 staged in LDS k-contiguous with
 // a 17-element row stride, and the next slice prefetched into registers
 // while the current one is multiplied.  TA / TB pick the global read order
-// that keeps the loads coalesced for each operand layout.
+// that keeps the loads coalesced for each operand layout.  Registers are
+// held to 256 per lane so that two workgroups share a CU.
 constexpr int H = 128, HK = 16, HP = HK + 1;
 
+// One thread's share of a 128 x HK slice of op(X)(i, k), X column-major
+// with leading dimension ld: KMAJOR when k is X's contiguous index
+// (op(X)(i, k) = X[k + ld i]), else i is (op(X)(i, k) = X[i + ld k]).
+// Each thread keeps one base pointer, a uniform stride and one bound, and
+// reads consecutive addresses with its neighbours.
+template <typename T, bool KMAJOR>
+struct SliceReader {
+  static constexpr int NR = H * HK / 256;
+  const T* p;
+  int64_t ld, step;
+  int ti, tk, lim;
+  T r[NR];
+  __device__ SliceReader(const T* x, int64_t ld_, int i0, int ni, int tid)
+      : ld(ld_) {
+    if (KMAJOR) {
+      constexpr int CS = 256 / HK;  // i between a thread's elements
+      tk = tid % HK;
+      ti = tid / HK;
+      p = x + (int64_t)(i0 + ti) * ld + tk;
+      step = (int64_t)CS * ld;
+      lim = min(NR, (ni - i0 - ti + CS - 1) / CS);  // elements in range
+    } else {
+      ti = tid % H;
+      tk = tid / H;  // k = tk + 2 q
+      p = x + i0 + ti + ld * tk;
+      step = 2 * ld;
+      lim = i0 + ti < ni;
+    }
+  }
+  __device__ void load(int k0, int K) {
+    if (KMAJOR) {
+      const bool kin = k0 + tk < K;
+#pragma unroll
+      for (int q = 0; q < NR; q++)
+        r[q] = (kin && q < lim) ? p[k0 + q * step] : (T)0;
+    } else {
+      const T* pk = p + (int64_t)k0 * ld;
+      const int nk = lim ? (K - k0 - tk + 1) >> 1 : 0;
+#pragma unroll
+      for (int q = 0; q < NR; q++) r[q] = q < nk ? pk[q * step] : (T)0;
+    }
+  }
+  __device__ void store(T (*S)[HP]) const {
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+      if (KMAJOR)
+        S[ti + q * (256 / HK)][tk] = r[q];
+      else
+        S[ti][tk + 2 * q] = r[q];
+    }
+  }
+};
+
 template <typename T, int TA, int TB>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 gemm128_kernel(const GemmDesc* __restrict__ desc, const T* __restrict__ a_all,
                const T* __restrict__ b_all, T* __restrict__ c_all) {
   const GemmDesc d = desc[blockIdx.y];
@@ -150,73 +204,38 @@ gemm128_kernel(const GemmDesc* __restrict__ desc, const T* __restrict__ a_all,
   __shared__ T Bs[H][HP];  // Bs[j][k] = op(B)(k0 + k, j0 + j)
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int wr = (wid >> 1) * 64, wc = (wid & 1) * 64;
-  // one global slice covers KL values of k: 128-byte segments along k
-  constexpr int KL = sizeof(T) == 4 ? 32 : 16, NR = H * KL / 256;
-  // element r of a thread's slice: op(A)(i, k) and op(B)(k, j) coordinates.
-  // TA = 0 reads along i (A column-major m x k), TA = 1 along k (A stored
-  // k x m); TB = 0 reads along k (B column-major k x n), TB = 1 along j.
-  auto acoord = [&](int r, int& i, int& k) {
-    const int e = tid + 256 * r;
-    i = TA ? e / KL : e % H;
-    k = TA ? e % KL : e / H;
-  };
-  auto bcoord = [&](int r, int& j, int& k) {
-    const int e = tid + 256 * r;
-    j = TB ? e % H : e / KL;
-    k = TB ? e / H : e % KL;
-  };
-  T ra[NR], rb[NR];
-  auto gload = [&](int k0) {
-#pragma unroll
-    for (int r = 0; r < NR; r++) {
-      int ai, ak, bj, bk;
-      acoord(r, ai, ak);
-      bcoord(r, bj, bk);
-      const int gi = i0 + ai, gk = k0 + ak;
-      ra[r] = (gi < d.m && gk < d.k)
-                  ? (TA ? A[gk + lda * gi] : A[gi + lda * gk])
-                  : (T)0;
-      const int gj = j0 + bj, gk2 = k0 + bk;
-      rb[r] = (gj < d.n && gk2 < d.k)
-                  ? (TB ? B[gj + ldb * gk2] : B[gk2 + ldb * gj])
-                  : (T)0;
-    }
-  };
+  SliceReader<T, TA == 1> ra(A, lda, i0, d.m, tid);
+  SliceReader<T, TB == 0> rb(B, ldb, j0, d.n, tid);
   typedef typename Mfma<T>::acc_t acc_t;
   acc_t acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
     for (int b = 0; b < 4; b++) acc[a][b] = acc_t{0, 0, 0, 0};
-  gload(0);
-  for (int k0 = 0; k0 < d.k; k0 += KL) {
+  ra.load(0, d.k);
+  rb.load(0, d.k);
+  for (int k0 = 0; k0 < d.k; k0 += HK) {
+    __syncthreads();
+    ra.store(As);
+    rb.store(Bs);
+    __syncthreads();
+    if (k0 + HK < d.k) {
+      ra.load(k0 + HK, d.k);
+      rb.load(k0 + HK, d.k);
+    }
 #pragma unroll
-    for (int h = 0; h < KL / HK; h++) {
-      __syncthreads();
+    for (int ks = 0; ks < HK; ks += 4) {
+      const int kr = ks + (lane >> 4);
+      T av[4], bv[4];
 #pragma unroll
-      for (int r = 0; r < NR; r++) {
-        int ai, ak, bj, bk;
-        acoord(r, ai, ak);
-        bcoord(r, bj, bk);
-        if (ak / HK == h) As[ai][ak % HK] = ra[r];
-        if (bk / HK == h) Bs[bj][bk % HK] = rb[r];
-      }
-      __syncthreads();
-      if (h == KL / HK - 1 && k0 + KL < d.k) gload(k0 + KL);
+      for (int a = 0; a < 4; a++) av[a] = As[wr + 16 * a + (lane & 15)][kr];
 #pragma unroll
-      for (int ks = 0; ks < HK; ks += 4) {
-        const int kr = ks + (lane >> 4);
-        T av[4], bv[4];
+      for (int b = 0; b < 4; b++) bv[b] = Bs[wc + 16 * b + (lane & 15)][kr];
 #pragma unroll
-        for (int a = 0; a < 4; a++) av[a] = As[wr + 16 * a + (lane & 15)][kr];
+      for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) bv[b] = Bs[wc + 16 * b + (lane & 15)][kr];
-#pragma unroll
-        for (int a = 0; a < 4; a++)
-#pragma unroll
-          for (int b = 0; b < 4; b++)
-            acc[a][b] = Mfma<T>::mac(bv[b], av[a], acc[a][b]);
-      }
+        for (int b = 0; b < 4; b++)
+          acc[a][b] = Mfma<T>::mac(bv[b], av[a], acc[a][b]);
     }
   }
   const T alpha = (T)d.alpha, beta = (T)d.beta;
@@ -819,14 +838,18 @@ scatter_kernel(const ScatterDesc* __restrict__ desc, const TS* __restrict__ src_
 
 // Tile-shape autotuner for the batched GEMM.  A product class is (element
 // size, ta, tb, ceil log2 of max m, max n and the batch count).  Its first
-// two launches run the 64 x 64 and the 128 x 128 kernel on the real data
-// (both compute the same product) under HIP events; every later launch of
-// the class uses the faster one.  Classes narrower than 256 rows or columns
-// use the 64 x 64 kernel untimed.  KI_GEMM_TILE = 64 / 128 forces a shape.
+// four launches alternate the 64 x 64 and the 128 x 128 kernel on the real
+// data (both compute the same product) under HIP events, each shape's time
+// being the faster of its two runs (the first run of a kernel can carry
+// one-time costs); every later launch of the class uses the faster shape.
+// Classes narrower than 256 rows or columns use the 64 x 64 kernel untimed.
+// KI_GEMM_TILE = 64 / 128 forces a shape.
 struct TileChoice {
   float ms[2] = {-1.f, -1.f};
+  int runs = 0;
   int best = -1;
 };
+constexpr int kTuneRuns = 4;
 
 static int ceil_log2(int x) {
   int l = 0;
@@ -873,7 +896,7 @@ void launch_gemm(const GemmDesc* d_desc, int ndesc, int max_m, int max_n,
     if (tc->best >= 0) {
       big = tc->best == 1;
     } else {
-      tuning = tc->ms[0] < 0 ? 0 : 1;
+      tuning = tc->runs & 1;
       big = tuning == 1;
     }
   }
@@ -913,9 +936,8 @@ void launch_gemm(const GemmDesc* d_desc, int ndesc, int max_m, int max_n,
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    tc->ms[tuning] = ms;
-    if (tc->ms[0] >= 0 && tc->ms[1] >= 0)
-      tc->best = tc->ms[1] < tc->ms[0] ? 1 : 0;
+    tc->ms[tuning] = tc->ms[tuning] < 0 ? ms : std::min(tc->ms[tuning], ms);
+    if (++tc->runs >= kTuneRuns) tc->best = tc->ms[1] < tc->ms[0] ? 1 : 0;
   }
 }
 

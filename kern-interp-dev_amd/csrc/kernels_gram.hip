@@ -103,10 +103,11 @@ gram_kernel(const GramDesc* __restrict__ desc, const TI* __restrict__ a_all,
 // (4 x 4 MFMA tiles each), K slices of 16 staged in LDS k-contiguous with a
 // 17-element row stride (conflict-free stores and operand reads), and the
 // next slice prefetched into registers while the current one is multiplied.
+// Registers are held to 256 per lane so that two workgroups share a CU.
 constexpr int G2 = 128, G2K = 16, G2P = G2K + 1;
 
 template <typename TI>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 gram128_kernel(const GramDesc* __restrict__ desc, const TI* __restrict__ a_all,
                double* __restrict__ g_all) {
   const GramDesc d = desc[blockIdx.y];
@@ -125,17 +126,25 @@ gram128_kernel(const GramDesc* __restrict__ desc, const TI* __restrict__ a_all,
   __shared__ double Bs[G2][G2P];  // Bs[j][k] = A(k0 + k, j0 + j)
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int wr = (wid >> 1) * 64, wc = (wid & 1) * 64;
-  // one global slice covers KL rows of A: 128 bytes per column segment
-  constexpr int KL = sizeof(TI) == 4 ? 32 : 16, NR = G2 * KL / 256;
+  // A slice is G2K rows of A.  Thread tid reads row k = tid % G2K of the
+  // tile's columns tid / G2K + 16 r: one base pointer per operand, a
+  // uniform column stride and one column-count bound per operand.  At two
+  // workgroups per CU the other workgroup's products cover this one's
+  // barriers and loads.
+  constexpr int NR = G2 * G2K / 256, CS = 256 / G2K;
+  const int kq = tid % G2K, cq = tid / G2K;
+  const TI* pa = A + (int64_t)(i0 + cq) * lda + kq;
+  const TI* pb = A + (int64_t)(j0 + cq) * lda + kq;
+  const int64_t cstep = (int64_t)CS * lda;
+  const int na = min(NR, (n - i0 - cq + CS - 1) / CS);  // valid columns
+  const int nb2 = min(NR, (n - j0 - cq + CS - 1) / CS);
   TI ra[NR], rb[NR];
   auto gload = [&](int k0) {
+    const bool kin = k0 + kq < m;
 #pragma unroll
     for (int r = 0; r < NR; r++) {
-      const int e = tid + 256 * r;
-      const int k = e % KL, i = e / KL;
-      const int gk = k0 + k, gi = i0 + i, gj = j0 + i;
-      ra[r] = (gk < m && gi < n) ? A[gk + lda * gi] : (TI)0;
-      rb[r] = (gk < m && gj < n) ? A[gk + lda * gj] : (TI)0;
+      ra[r] = (kin && r < na) ? pa[k0 + r * cstep] : (TI)0;
+      rb[r] = (kin && r < nb2) ? pb[k0 + r * cstep] : (TI)0;
     }
   };
   f64x4 acc[4][4];
@@ -144,35 +153,28 @@ gram128_kernel(const GramDesc* __restrict__ desc, const TI* __restrict__ a_all,
 #pragma unroll
     for (int b = 0; b < 4; b++) acc[a][b] = f64x4{0, 0, 0, 0};
   gload(0);
-  for (int k0 = 0; k0 < m; k0 += KL) {
+  for (int k0 = 0; k0 < m; k0 += G2K) {
+    __syncthreads();
 #pragma unroll
-    for (int h = 0; h < KL / G2K; h++) {
-      __syncthreads();
+    for (int r = 0; r < NR; r++) {
+      As[cq + r * CS][kq] = (double)ra[r];
+      Bs[cq + r * CS][kq] = (double)rb[r];
+    }
+    __syncthreads();
+    if (k0 + G2K < m) gload(k0 + G2K);
 #pragma unroll
-      for (int r = 0; r < NR; r++) {
-        const int e = tid + 256 * r;
-        const int k = e % KL, i = e / KL;
-        if (k / G2K == h) {
-          As[i][k % G2K] = (double)ra[r];
-          Bs[i][k % G2K] = (double)rb[r];
-        }
-      }
-      __syncthreads();
-      if (h == KL / G2K - 1 && k0 + KL < m) gload(k0 + KL);
+    for (int ks = 0; ks < G2K; ks += 4) {
+      const int kr = ks + (lane >> 4);
+      double av[4], bv[4];
 #pragma unroll
-      for (int ks = 0; ks < G2K; ks += 4) {
-        const int kr = ks + (lane >> 4);
-        double av[4], bv[4];
+      for (int a = 0; a < 4; a++) av[a] = As[wr + 16 * a + (lane & 15)][kr];
 #pragma unroll
-        for (int a = 0; a < 4; a++) av[a] = As[wr + 16 * a + (lane & 15)][kr];
+      for (int b = 0; b < 4; b++) bv[b] = Bs[wc + 16 * b + (lane & 15)][kr];
 #pragma unroll
-        for (int b = 0; b < 4; b++) bv[b] = Bs[wc + 16 * b + (lane & 15)][kr];
+      for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int a = 0; a < 4; a++)
-#pragma unroll
-          for (int b = 0; b < 4; b++)
-            acc[a][b] = Mfma<double>::mac(bv[b], av[a], acc[a][b]);
-      }
+        for (int b = 0; b < 4; b++)
+          acc[a][b] = Mfma<double>::mac(bv[b], av[a], acc[a][b]);
     }
   }
   double* G = g_all + d.g_off;
