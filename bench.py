@@ -137,6 +137,24 @@ def cpu_baseline(G, index, budget_s=20.0):
                                                    pb.id_tol)}
 
 
+COPY_BYTES = 1 << 30
+
+
+def calibrate_copy():
+    """One elementwise kernel that reads COPY_BYTES and writes COPY_BYTES:
+    what rocprofv3's FETCH_SIZE / WRITE_SIZE report for it turns their
+    readings for the solver's kernels into bytes."""
+    import torch
+    x = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    torch.mul(x, 2.0, out=y)
+    torch.cuda.synchronize()
+    assert float(y[-1]) == 2.0
+    print(json.dumps({"copy_kernel_bytes_read": COPY_BYTES,
+                      "copy_kernel_bytes_written": COPY_BYTES}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,7 +166,13 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-predictor", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a 1 GiB read + 1 GiB write elementwise "
+                    "kernel (scale for the HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the solver's multi-GPU "
               "sharding is a design, not part of this build", file=sys.stderr)

@@ -422,52 +422,100 @@ lu_pstep(T* __restrict__ A, int ld, int m, int j0, int jb,
 // The whole column-panel factorization A[j0:m, j0:j0+jb] of one tall matrix in
 // a single cooperative launch.  Workgroup b keeps rows [b*PR, b*PR + PR) of
 // the panel in LDS for all jb column steps.  Per column c: every workgroup
-// posts its best pivot candidate (|value|, row, and the row's panel entries)
-// and the owner of row c posts row c; one grid barrier; then every workgroup
-// picks the same pivot p (first row of maximal |a|, as idamax), the owners of
-// rows c and p exchange them, and each workgroup applies the rank-1 update to
-// its rows below c.  Posted data goes to per-step slots, so no slot is
-// rewritten while another workgroup may still read it.  Same interchanges as
-// the one-workgroup panel (LAPACK getf2 order).
+// posts its best pivot candidate (row, |value| and the row's panel entries)
+// and the owner of row c posts row c; then every workgroup reads all
+// candidates and picks the same pivot p (first row of maximal |a|, as
+// idamax), the owners of rows c and p exchange them, and each workgroup
+// applies the rank-1 update to its rows below c.  Same interchanges as the
+// one-workgroup panel (LAPACK getf2 order).
 //
-// The grid barrier is one arrival counter per launch (zeroed by the host
-// before the panels run): each workgroup adds 1 after its posts (release),
-// then waits for (c + 1) * nwg arrivals (acquire).  The cooperative launch
-// guarantees that every workgroup is resident; the wait is still bounded,
-// and a workgroup that gives up sets *fault so the host reports an error
-// instead of hanging.
+// Exchange without barriers or cache-wide fences: every posted word is a
+// 64-bit relaxed agent-scope atomic holding 32 data bits and a 32-bit tag
+// (tag0 + c, unique per launch and step; the host zeroes the buffer once
+// and never reuses a tag), and a reader spins on each word it needs until
+// the tag matches.  Per-location coherence alone makes the data valid, so
+// no write-back / invalidate of the L2s is needed per column.  Every wait
+// is bounded: a workgroup that gives up sets *fault (the others then stop
+// waiting) and the host reports an error instead of hanging.
 constexpr int PR = 224;  // panel rows per workgroup (LDS: PR x LB values)
-constexpr long long kBarrierSpins = 1ll << 26;
+constexpr long long kExchangeSpins = 1ll << 24;
 
-__device__ __forceinline__ void coop_barrier(unsigned* count, unsigned target,
-                                             int* fault) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    long long spins = 0;
-    while (__hip_atomic_load(count, __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT) < target) {
-      __builtin_amdgcn_s_sleep(1);
-      if (++spins > kBarrierSpins) {
+__device__ __forceinline__ void tag_put(uint64_t* p, uint32_t bits,
+                                        uint32_t tag) {
+  __hip_atomic_store(p, ((uint64_t)tag << 32) | bits, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t tag_get(uint64_t* p, uint32_t tag,
+                                            int* fault) {
+  for (long long spins = 0;; spins++) {
+    const uint64_t w =
+        __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((uint32_t)(w >> 32) == tag) return (uint32_t)w;
+    if ((spins & 255) == 255) {
+      if (__hip_atomic_load(fault, __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_AGENT) != 0)
+        return 0;
+      if (spins > kExchangeSpins) {
         __hip_atomic_store(fault, 1, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
-        break;
+        return 0;
       }
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __builtin_amdgcn_s_sleep(1);
   }
-  __syncthreads();
 }
+
+// a value as tagged words: one for float, two (low, high) for double
+template <typename T>
+struct Tagged;
+template <>
+struct Tagged<float> {
+  static constexpr int W = 1;
+  __device__ static void put(uint64_t* p, float v, uint32_t tag) {
+    tag_put(p, __float_as_uint(v), tag);
+  }
+  __device__ static float get(uint64_t* p, uint32_t tag, int* fault) {
+    return __uint_as_float(tag_get(p, tag, fault));
+  }
+};
+template <>
+struct Tagged<double> {
+  static constexpr int W = 2;
+  __device__ static void put(uint64_t* p, double v, uint32_t tag) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    tag_put(p, (uint32_t)b, tag);
+    tag_put(p + 1, (uint32_t)(b >> 32), tag);
+  }
+  __device__ static double get(uint64_t* p, uint32_t tag, int* fault) {
+    const uint64_t lo = tag_get(p, tag, fault);
+    const uint64_t hi = tag_get(p + 1, tag, fault);
+    return __longlong_as_double((long long)(lo | (hi << 32)));
+  }
+};
+
+// Exchange words of one matrix: per column step, wg_max candidate records
+// (row index, |value|, LB row entries) followed by row c's LB entries.
+template <typename T>
+struct PanelXch {
+  static constexpr int W = Tagged<T>::W;
+  static constexpr int kRec = 1 + W + LB * W;
+  __host__ __device__ static int64_t step_words(int wg_max) {
+    return (int64_t)wg_max * kRec + LB * W;
+  }
+  __host__ __device__ static int64_t words(int wg_max) {
+    return LB * step_words(wg_max);
+  }
+};
 
 template <typename T>
 __global__ void __launch_bounds__(256)
 lu_coop_panel(const MatDesc* __restrict__ desc, T* __restrict__ a_all,
               int* __restrict__ piv_all, int j0, int jend, int wg_max,
-              T* __restrict__ cv_all, int* __restrict__ ci_all,
-              T* __restrict__ crow_all, T* __restrict__ rowc_all,
-              unsigned* __restrict__ bar_all, int* __restrict__ fault) {
+              uint64_t* __restrict__ xch_all, uint32_t tag0,
+              int* __restrict__ fault) {
+  typedef Tagged<T> TW;
+  typedef PanelXch<T> X;
   // matrix blockIdx.y of the batch; its panel is columns [j0, j0 + jb) with
   // jb limited by min(m, n) and by jend (the enclosing outer panel's end)
   const int box = blockIdx.y;
@@ -481,11 +529,7 @@ lu_coop_panel(const MatDesc* __restrict__ desc, T* __restrict__ a_all,
   const int mr = m - j0;
   const int nwg = (mr + PR - 1) / PR, b = blockIdx.x, tid = threadIdx.x;
   if (b >= nwg) return;  // this matrix needs fewer workgroups
-  T* cv = cv_all + (int64_t)box * LB * wg_max;
-  int* ci = ci_all + (int64_t)box * LB * wg_max;
-  T* crow = crow_all + (int64_t)box * LB * wg_max * LB;
-  T* rowc = rowc_all + (int64_t)box * LB * LB;
-  unsigned* bar = bar_all + box;
+  uint64_t* xch = xch_all + (int64_t)box * X::words(wg_max);
   const int r0 = b * PR;
   const int nr = min(PR, mr - r0);
   __shared__ T P[LB][PR + 1];  // P[q][i] = A[j0 + r0 + i, j0 + q]
@@ -501,6 +545,9 @@ lu_coop_panel(const MatDesc* __restrict__ desc, T* __restrict__ a_all,
   __syncthreads();
   const int lane = tid & 63, wid = tid >> 6;
   for (int c = 0; c < jb; c++) {
+    const uint32_t tag = tag0 + (uint32_t)c;
+    uint64_t* step = xch + (int64_t)c * X::step_words(wg_max);
+    uint64_t* rowc = step + (int64_t)wg_max * X::kRec;
     // local candidate: first local row of maximal |a| among rows >= c
     T best = -1;
     int bi = INT_MAX;
@@ -531,24 +578,25 @@ lu_coop_panel(const MatDesc* __restrict__ desc, T* __restrict__ a_all,
           ii = ri[w];
         }
       s_p = ii;
-      cv[(int64_t)c * nwg + b] = bb;
-      ci[(int64_t)c * nwg + b] = ii;
+      uint64_t* rec = step + (int64_t)b * X::kRec;
+      tag_put(rec, (uint32_t)ii, tag);
+      TW::put(rec + 1, bb, tag);
     }
     __syncthreads();
     const int mine = s_p;
     if (mine != INT_MAX && tid < jb)
-      crow[((int64_t)c * nwg + b) * LB + tid] = P[tid][mine - r0];
+      TW::put(step + (int64_t)b * X::kRec + 1 + X::W + X::W * tid,
+              P[tid][mine - r0], tag);
     if (c >= r0 && c < r0 + nr && tid < jb)
-      rowc[(int64_t)c * LB + tid] = P[tid][c - r0];
-    __threadfence();
-    coop_barrier(bar, (unsigned)(c + 1) * nwg, fault);
+      TW::put(rowc + X::W * tid, P[tid][c - r0], tag);
     // global pivot: the same reduction order in every workgroup
     if (tid < 64) {
       T bb = -1;
       int ii = INT_MAX, ww = -1;
       for (int w = tid; w < nwg; w += 64) {
-        const T v = cv[(int64_t)c * nwg + w];
-        const int x = ci[(int64_t)c * nwg + w];
+        uint64_t* rec = step + (int64_t)w * X::kRec;
+        const int x = (int)tag_get(rec, tag, fault);
+        const T v = TW::get(rec + 1, tag, fault);
         if (v > bb || (v == bb && x < ii)) {
           bb = v;
           ii = x;
@@ -566,10 +614,15 @@ lu_coop_panel(const MatDesc* __restrict__ desc, T* __restrict__ a_all,
         }
       }
       // no finite candidate (or all zero entries tie at row c): keep row c
-      int p = (ii == INT_MAX || ii < c || ii >= mr || !(bb > 0)) ? c : ii;
+      int p = (ii == INT_MAX || ii < c || ii >= mr || !(bb > 0) || ww < 0)
+                  ? c
+                  : ii;
       if (tid < jb)
-        prow[tid] = p == c ? rowc[(int64_t)c * LB + tid]
-                           : crow[((int64_t)c * nwg + ww) * LB + tid];
+        prow[tid] =
+            p == c ? TW::get(rowc + X::W * tid, tag, fault)
+                   : TW::get(step + (int64_t)ww * X::kRec + 1 + X::W +
+                                 X::W * tid,
+                             tag, fault);
       if (tid == 0) {
         s_p = p;
         if (b == 0) piv[j0 + c] = j0 + p;
@@ -577,10 +630,12 @@ lu_coop_panel(const MatDesc* __restrict__ desc, T* __restrict__ a_all,
     }
     __syncthreads();
     const int p = s_p;
-    // interchange rows c and p
+    // interchange rows c and p (row c's entries: the posted copy)
     if (p != c && tid < jb) {
       if (p >= r0 && p < r0 + nr)
-        P[tid][p - r0] = rowc[(int64_t)c * LB + tid];
+        P[tid][p - r0] = c >= r0 && c < r0 + nr
+                             ? P[tid][c - r0]
+                             : TW::get(rowc + X::W * tid, tag, fault);
       if (c >= r0 && c < r0 + nr) P[tid][c - r0] = prow[tid];
     }
     __syncthreads();
@@ -1003,9 +1058,7 @@ template <typename T>
 struct CoopPanels {
   int nb = 0, wg_max = 0, npanels = 0, panel_idx = 0;
   MatDesc* d_desc = nullptr;
-  T *cv = nullptr, *crow = nullptr, *rowc = nullptr;
-  int* ci = nullptr;
-  unsigned* bar = nullptr;  // one arrival counter per (panel, matrix)
+  uint64_t* xch = nullptr;  // tagged exchange words, zeroed once
   int* fault = nullptr;
 
   // false (and nothing allocated) when the device cannot keep every
@@ -1031,19 +1084,16 @@ struct CoopPanels {
         &per_cu, reinterpret_cast<const void*>(&lu_coop_panel<T>), 256, 0));
     if (!can || (int64_t)per_cu * ncu < (int64_t)wg_max * nb) return false;
     npanels = (maxmn + LB - 1) / LB;
-    const size_t slots = (size_t)LB * wg_max * nb;
+    // tags are 1 + panel * LB + column: unique while they fit in 32 bits
+    KI_CHECK((int64_t)npanels * LB < (1ll << 31), 2,
+             "too many LU panels for the exchange tags");
+    const size_t words = (size_t)PanelXch<T>::words(wg_max) * nb + 1;
     KI_HIP(hipMallocAsync((void**)&d_desc, sizeof(MatDesc) * nb, s));
     KI_HIP(hipMemcpyAsync(d_desc, h.data(), sizeof(MatDesc) * nb,
                           hipMemcpyHostToDevice, s));
-    KI_HIP(hipMallocAsync((void**)&cv, sizeof(T) * slots, s));
-    KI_HIP(hipMallocAsync((void**)&ci, sizeof(int) * slots, s));
-    KI_HIP(hipMallocAsync((void**)&crow, sizeof(T) * slots * LB, s));
-    KI_HIP(hipMallocAsync((void**)&rowc, sizeof(T) * (size_t)LB * LB * nb,
-                          s));
-    const size_t nbar = (size_t)npanels * nb + 1;
-    KI_HIP(hipMallocAsync((void**)&bar, sizeof(unsigned) * nbar, s));
-    KI_HIP(hipMemsetAsync(bar, 0, sizeof(unsigned) * nbar, s));
-    fault = reinterpret_cast<int*>(bar + nbar - 1);
+    KI_HIP(hipMallocAsync((void**)&xch, sizeof(uint64_t) * words, s));
+    KI_HIP(hipMemsetAsync(xch, 0, sizeof(uint64_t) * words, s));
+    fault = reinterpret_cast<int*>(xch + words - 1);
     KI_HIP(hipStreamSynchronize(s));  // h may die after return
     return true;
   }
@@ -1053,9 +1103,9 @@ struct CoopPanels {
   // the caller then factors this and the remaining panels another way.
   bool panel(T* a, int* piv_all, int j0, int jend, hipStream_t s) {
     KI_CHECK(panel_idx < npanels, 2, "cooperative panel count exceeded");
-    unsigned* pbar = bar + (size_t)panel_idx * nb;
-    void* args[] = {&d_desc, &a,   &piv_all, &j0,   &jend, &wg_max,
-                    &cv,     &ci,  &crow,    &rowc, &pbar, &fault};
+    uint32_t tag0 = 1u + (uint32_t)panel_idx * LB;
+    void* args[] = {&d_desc, &a,   &piv_all, &j0,   &jend,
+                    &wg_max, &xch, &tag0,    &fault};
     const hipError_t e = hipLaunchCooperativeKernel(
         reinterpret_cast<const void*>(&lu_coop_panel<T>), dim3(wg_max, nb),
         dim3(256), args, 0, s);
@@ -1073,12 +1123,8 @@ struct CoopPanels {
     KI_HIP(hipMemcpyAsync(&hf, fault, sizeof(int), hipMemcpyDeviceToHost, s));
     KI_HIP(hipStreamSynchronize(s));
     KI_HIP(hipFreeAsync(d_desc, s));
-    KI_HIP(hipFreeAsync(cv, s));
-    KI_HIP(hipFreeAsync(ci, s));
-    KI_HIP(hipFreeAsync(crow, s));
-    KI_HIP(hipFreeAsync(rowc, s));
-    KI_HIP(hipFreeAsync(bar, s));
-    KI_CHECK(hf == 0, 3, "LU panel grid barrier timed out");  // device error
+    KI_HIP(hipFreeAsync(xch, s));
+    KI_CHECK(hf == 0, 3, "LU panel exchange timed out");  // device error
   }
 };
 
