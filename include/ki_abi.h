@@ -62,7 +62,11 @@ typedef struct ki_params {
                                over many workgroups (0: default, 1024)        */
   int solve_graph;          /* 1: capture the factor apply once per
                                factorization in a HIP graph and replay it    */
-  int reserved[5];
+  int chol_coop;            /* 1: the Gram ID's large-box pivoted Cholesky
+                               runs a block of pivot steps per cooperative
+                               launch when all its workgroups fit; 0: one
+                               launch per pivot step                          */
+  int reserved[4];
 } ki_params;
 
 void ki_params_default(ki_params* p);

@@ -55,7 +55,8 @@ class KiParams(C.Structure):
         ("lu_wide_rows", C.c_int),
         ("lu_coop_rows", C.c_int),
         ("solve_graph", C.c_int),
-        ("reserved", C.c_int * 5),
+        ("chol_coop", C.c_int),
+        ("reserved", C.c_int * 4),
     ]
 
 

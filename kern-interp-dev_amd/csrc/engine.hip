@@ -617,7 +617,8 @@ void skeletonize_t(Ctx& c) {
           upload(c.cdesc, large, s);
           pchol_large(large, c.cdesc.as<CholDesc>(), c.gram.as<double>(),
                       c.cws.as<double>(), c.qpiv.as<int>(),
-                      c.cstate.as<int>(), c.qrank.as<int>(), tol, s);
+                      c.cstate.as<int>(), c.qrank.as<int>(), tol,
+                      c.prm.chol_coop != 0, s);
         }
         c.clock("pivoted Cholesky (pchol_wg, pchol_step)", tch.stop(), 0., 0.,
                 0);
@@ -1496,6 +1497,7 @@ void ki_params_default(ki_params* p) {
   p->id_method = 0;
   p->fp32_tol_floor = 1e-6;
   p->solve_graph = 1;
+  p->chol_coop = 1;
 }
 
 const char* ki_last_error(void) { return ki::g_err.c_str(); }

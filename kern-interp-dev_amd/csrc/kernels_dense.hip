@@ -20,6 +20,7 @@
 
 #include "ki_internal.h"
 #include "mfma.h"
+#include "xch.h"
 
 namespace ki {
 
@@ -429,70 +430,8 @@ lu_pstep(T* __restrict__ A, int ld, int m, int j0, int jb,
 // applies the rank-1 update to its rows below c.  Same interchanges as the
 // one-workgroup panel (LAPACK getf2 order).
 //
-// Exchange without barriers or cache-wide fences: every posted word is a
-// 64-bit relaxed agent-scope atomic holding 32 data bits and a 32-bit tag
-// (tag0 + c, unique per launch and step; the host zeroes the buffer once
-// and never reuses a tag), and a reader spins on each word it needs until
-// the tag matches.  Per-location coherence alone makes the data valid, so
-// no write-back / invalidate of the L2s is needed per column.  Every wait
-// is bounded: a workgroup that gives up sets *fault (the others then stop
-// waiting) and the host reports an error instead of hanging.
+// The exchange uses tagged words (xch.h), tag tag0 + c for step c.
 constexpr int PR = 224;  // panel rows per workgroup (LDS: PR x LB values)
-constexpr long long kExchangeSpins = 1ll << 24;
-
-__device__ __forceinline__ void tag_put(uint64_t* p, uint32_t bits,
-                                        uint32_t tag) {
-  __hip_atomic_store(p, ((uint64_t)tag << 32) | bits, __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ uint32_t tag_get(uint64_t* p, uint32_t tag,
-                                            int* fault) {
-  for (long long spins = 0;; spins++) {
-    const uint64_t w =
-        __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if ((uint32_t)(w >> 32) == tag) return (uint32_t)w;
-    if ((spins & 255) == 255) {
-      if (__hip_atomic_load(fault, __ATOMIC_RELAXED,
-                            __HIP_MEMORY_SCOPE_AGENT) != 0)
-        return 0;
-      if (spins > kExchangeSpins) {
-        __hip_atomic_store(fault, 1, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-        return 0;
-      }
-    }
-    __builtin_amdgcn_s_sleep(1);
-  }
-}
-
-// a value as tagged words: one for float, two (low, high) for double
-template <typename T>
-struct Tagged;
-template <>
-struct Tagged<float> {
-  static constexpr int W = 1;
-  __device__ static void put(uint64_t* p, float v, uint32_t tag) {
-    tag_put(p, __float_as_uint(v), tag);
-  }
-  __device__ static float get(uint64_t* p, uint32_t tag, int* fault) {
-    return __uint_as_float(tag_get(p, tag, fault));
-  }
-};
-template <>
-struct Tagged<double> {
-  static constexpr int W = 2;
-  __device__ static void put(uint64_t* p, double v, uint32_t tag) {
-    const uint64_t b = (uint64_t)__double_as_longlong(v);
-    tag_put(p, (uint32_t)b, tag);
-    tag_put(p + 1, (uint32_t)(b >> 32), tag);
-  }
-  __device__ static double get(uint64_t* p, uint32_t tag, int* fault) {
-    const uint64_t lo = tag_get(p, tag, fault);
-    const uint64_t hi = tag_get(p + 1, tag, fault);
-    return __longlong_as_double((long long)(lo | (hi << 32)));
-  }
-};
 
 // Exchange words of one matrix: per column step, wg_max candidate records
 // (row index, |value|, LB row entries) followed by row c's LB entries.
@@ -594,9 +533,10 @@ lu_coop_panel(const MatDesc* __restrict__ desc, T* __restrict__ a_all,
       T bb = -1;
       int ii = INT_MAX, ww = -1;
       for (int w = tid; w < nwg; w += 64) {
-        uint64_t* rec = step + (int64_t)w * X::kRec;
-        const int x = (int)tag_get(rec, tag, fault);
-        const T v = TW::get(rec + 1, tag, fault);
+        uint32_t wv[1 + X::W];
+        tag_get_seq<1 + X::W>(step + (int64_t)w * X::kRec, tag, fault, wv);
+        const int x = (int)wv[0];
+        const T v = TW::from(wv + 1);
         if (v > bb || (v == bb && x < ii)) {
           bb = v;
           ii = x;
@@ -1046,6 +986,9 @@ thread_local int g_lu_wide_rows = 0, g_lu_coop_rows = 0;
 // LUs of this process then use the per-column / one-workgroup panels.
 bool g_coop_refused = false;
 }  // namespace
+
+bool coop_refused() { return g_coop_refused; }
+void note_coop_refused() { g_coop_refused = true; }
 
 void set_lu_thresholds(int wide_rows, int coop_rows) {
   g_lu_wide_rows = wide_rows;

@@ -16,11 +16,13 @@
 //   pchol_wg      left-looking pivoted Cholesky, one workgroup per box
 //   pchol_step    one pivot step for a few large boxes, many workgroups
 //                 per box (the host loops over steps)
+//   pchol_coop    the same steps, a block of them per cooperative launch
 #include <algorithm>
 #include <cmath>
 
 #include "ki_internal.h"
 #include "mfma.h"
+#include "xch.h"
 
 namespace ki {
 
@@ -391,6 +393,155 @@ pchol_step(const CholDesc* __restrict__ desc, const double* __restrict__ g_all,
   dout[j] = fmax(dj - v * v, 0.0);
 }
 
+// The same pivot steps i0..i1-1 (one block of CB) for every large box in
+// one cooperative launch: one workgroup per 64-column chunk of a box
+// (chunks[] lists (box, chunk) pairs; every workgroup resident).  Lane l of
+// chunk b owns column j = 64 b + l, and every wave keeps the same copy of
+// its Schur diagonal in a register.  Per step the chunk posts its best
+// column (first index of the largest diagonal > -1/2, pchol_step's argmax)
+// and reads every chunk's, so all chunks pick the same pivot p; it reads
+// row p of the block's L columns from the tagged mirror Lt (written once
+// per launch by p's owner); and it computes its columns of L(:, i) exactly
+// as pchol_step does (the four waves split the sum over l, combined as
+// (s0 + s1) + (s2 + s3)), so both forms give bit-identical factors.
+// Exchange words (xch.h) carry the launch's tag:
+//   xch per box: CB steps x wg_max chunks x (column, value lo, value hi)
+//   lt  per box: CB x n tagged doubles, Lt(l - i0, j) = L(j, l)
+// At the end each chunk writes its diagonal to both halves of the double
+// buffer, where the next block (either form) and pchol_order read it.
+constexpr int kCholChunk = 64;
+
+__global__ void __launch_bounds__(256)
+pchol_coop(const CholDesc* __restrict__ desc, const int2* __restrict__ chunks,
+           const double* __restrict__ g_all, double* __restrict__ ws_all,
+           int* __restrict__ piv_all, int* __restrict__ state, int i0, int i1,
+           int wg_max, uint64_t* __restrict__ xch_all,
+           const int64_t* __restrict__ lt_off, uint64_t* __restrict__ lt_all,
+           uint32_t tag, double tol, int* __restrict__ fault) {
+  typedef Tagged<double> TW;
+  const int2 cb = chunks[blockIdx.x];
+  const int box = cb.x, b = cb.y;
+  const CholDesc d = desc[box];
+  const int n = d.n;
+  const int nwg = (n + kCholChunk - 1) / kCholChunk;
+  int* st = state + 4 * d.slot;  // [rank, done, -, -]
+  if (st[1]) return;
+  const int kend = min(d.kmax, n);
+  const double* G = g_all + d.g_off;
+  double* L = ws_all + d.l_off;
+  double* dg = ws_all + d.d_off;
+  int* piv = piv_all + d.piv_off;
+  uint64_t* xch = xch_all + (int64_t)box * CB * wg_max * 3;
+  uint64_t* lt = lt_all + lt_off[box];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int j = b * kCholChunk + lane;
+  double dj = j < n ? dg[j] : -1.0;
+  double thresh = i0 > 0 ? dg[2 * n] : 0.0;
+  __shared__ double Lp[CB];
+  __shared__ double red[4][64];
+  __shared__ double s_best;
+  __shared__ int s_p;
+  for (int i = i0; i < i1 && i < kend; i++) {
+    const int s = i - i0;
+    uint64_t* step = xch + (int64_t)s * wg_max * 3;
+    if (wid == 0) {
+      double best = -0.5;
+      int bi = n;
+      if (j < n && dj > best) {
+        best = dj;
+        bi = j;
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(best, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (ov > best || (ov == best && oi < bi)) {
+          best = ov;
+          bi = oi;
+        }
+      }
+      if (lane == 0) {
+        tag_put(step + 3 * b, (uint32_t)bi, tag);
+        TW::put(step + 3 * b + 1, best, tag);
+      }
+      double gb = -0.5;
+      int gi = n;
+      for (int w = lane; w < nwg; w += 64) {
+        uint32_t wv[3];
+        tag_get_seq<3>(step + 3 * w, tag, fault, wv);
+        const int x = (int)wv[0];
+        const double v = TW::from(wv + 1);
+        if (v > gb || (v == gb && x < gi)) {
+          gb = v;
+          gi = x;
+        }
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(gb, o, 64);
+        const int oi = __shfl_xor(gi, o, 64);
+        if (ov > gb || (ov == gb && oi < gi)) {
+          gb = ov;
+          gi = oi;
+        }
+      }
+      if (lane == 0) {
+        s_best = gb;
+        s_p = gi;
+      }
+    }
+    __syncthreads();
+    const double best = s_best;
+    const int p = s_p;
+    if (p < 0 || p >= n) break;  // nothing left (as pchol_step)
+    const double rii = sqrt(fmax(best, 0.0));
+    if (i == 0) thresh = fabs(tol * rii);
+    const bool crit = i > 0 && rii < thresh;
+    if (crit || rii == 0.0) {
+      if (b == 0 && tid == 0) {
+        st[0] = crit ? i : 0;
+        st[1] = 1;
+      }
+      break;
+    }
+    if (b == 0 && tid == 0) {
+      piv[i] = p;
+      if (i == 0) dg[2 * n] = thresh;
+    }
+    if (tid < s) Lp[tid] = TW::get(lt + 2 * ((int64_t)tid * n + p), tag, fault);
+    __syncthreads();
+    const bool live = j < n && dj >= 0.0 && j != p;
+    double part = 0.0;
+    if (live)
+      for (int l = i0 + wid; l < i; l += 4)
+        part += L[j + (int64_t)n * l] * Lp[l - i0];
+    red[wid][lane] = part;
+    __syncthreads();
+    if (j < n) {
+      double v;
+      if (dj < 0.0) {  // selected earlier: zero keeps the block GEMM finite
+        v = 0.0;
+      } else if (j == p) {
+        v = rii;
+        dj = -1.0;
+      } else {
+        const double sm =
+            G[j + (int64_t)n * p] -
+            ((red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]));
+        v = sm / rii;
+        dj = fmax(dj - v * v, 0.0);
+      }
+      if (wid == 0) {
+        L[j + (int64_t)n * i] = v;
+        TW::put(lt + 2 * ((int64_t)s * n + j), v, tag);
+      }
+    }
+    __syncthreads();  // red and Lp are rewritten by the next step
+  }
+  if (wid == 0 && j < n) {
+    dg[j] = dj;
+    dg[n + j] = dj;
+  }
+}
+
 // Initialise the large-box state: d = diag(G) in the even buffer.
 __global__ void pchol_init(const CholDesc* __restrict__ desc,
                            const double* __restrict__ g_all,
@@ -496,7 +647,7 @@ void pchol_small(const CholDesc* d_desc, int ndesc, double* g, double* ws,
 
 void pchol_large(const std::vector<CholDesc>& h, const CholDesc* d_desc,
                  double* g, double* ws, int* piv, int* state, int* rank,
-                 double tol, hipStream_t s) {
+                 double tol, bool coop, hipStream_t s) {
   const int nb = h.size();
   if (nb == 0) return;
   int maxn = 0, maxsteps = 0;
@@ -529,40 +680,121 @@ void pchol_large(const std::vector<CholDesc>& h, const CholDesc* d_desc,
   KI_HIP(hipMallocAsync((void**)&d_g, sizeof(GemmDesc) * gd.size(), s));
   KI_HIP(hipMemcpyAsync(d_g, gd.data(), sizeof(GemmDesc) * gd.size(),
                         hipMemcpyHostToDevice, s));
+  // cooperative form: every chunk of every box resident at once
+  const int wg_max = (maxn + kCholChunk - 1) / kCholChunk;
+  std::vector<int2> chunk;
+  for (int b = 0; b < nb; b++)
+    for (int q = 0; q * kCholChunk < h[b].n; q++) chunk.push_back({b, q});
+  const int nchunk = chunk.size();
+  if (coop) {
+    int dev = 0, ncu = 0, can = 0, per_cu = 0;
+    KI_HIP(hipGetDevice(&dev));
+    KI_HIP(hipDeviceGetAttribute(&can, hipDeviceAttributeCooperativeLaunch,
+                                 dev));
+    KI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                                 dev));
+    KI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(&pchol_coop), 256, 0));
+    coop = can && !coop_refused() && (int64_t)per_cu * ncu >= nchunk;
+  }
+  uint64_t* xch = nullptr;
+  uint64_t* lt = nullptr;
+  int64_t* d_lt_off = nullptr;
+  int2* d_chunk = nullptr;
+  int* fault = nullptr;
+  if (coop) {
+    std::vector<int64_t> lo(nb);
+    int64_t lt_words = 0;
+    for (int b = 0; b < nb; b++) {
+      lo[b] = lt_words;
+      lt_words += 2 * (int64_t)CB * h[b].n;
+    }
+    const size_t xw = (size_t)nb * CB * wg_max * 3 + 1;
+    KI_HIP(hipMallocAsync((void**)&xch, sizeof(uint64_t) * xw, s));
+    KI_HIP(hipMallocAsync((void**)&lt, sizeof(uint64_t) * lt_words, s));
+    KI_HIP(hipMallocAsync((void**)&d_lt_off, sizeof(int64_t) * nb, s));
+    KI_HIP(hipMallocAsync((void**)&d_chunk, sizeof(int2) * nchunk, s));
+    KI_HIP(hipMemcpyAsync(d_chunk, chunk.data(), sizeof(int2) * nchunk,
+                          hipMemcpyHostToDevice, s));
+    KI_HIP(hipMemsetAsync(xch, 0, sizeof(uint64_t) * xw, s));
+    KI_HIP(hipMemsetAsync(lt, 0, sizeof(uint64_t) * lt_words, s));
+    KI_HIP(hipMemcpyAsync(d_lt_off, lo.data(), sizeof(int64_t) * nb,
+                          hipMemcpyHostToDevice, s));
+    fault = reinterpret_cast<int*>(xch + xw - 1);
+    KI_HIP(hipStreamSynchronize(s));  // lo dies with this scope
+  }
   std::vector<int> hst(4 * (size_t)nb);
-  int i = 0;
-  for (; i < maxsteps; i++) {
-    const int i0 = i - i % CB;
-    hipLaunchKernelGGL(pchol_step, grid, dim3(256), 0, s, d_desc, g, ws, piv,
-                       state, i, i0, i & 1, tol);
-    if (i % CB == CB - 1 && i + 1 < maxsteps) {
-      // S -= L_blk L_blk^T is a plain GEMM per box: for a few large boxes
-      // the library's tuned kernel, else one batched launch
-      const size_t base = (size_t)(i / CB) * nb;
-      bool lib = nb <= 128 && maxn >= 512, any = false;
-      for (int b = 0; lib && b < nb; b++) {
-        const GemmDesc& q = gd[base + b];
-        if (q.m == 0) continue;
-        if (library_gemm<double>(q, ws, ws, g, s)) {
-          any = true;
-        } else {
-          KI_CHECK(!any, 3, "rocBLAS GEMM failed after earlier successes");
-          lib = false;
+  auto all_done = [&]() {
+    KI_HIP(hipMemcpyAsync(hst.data(), state, sizeof(int) * 4 * nb,
+                          hipMemcpyDeviceToHost, s));
+    KI_HIP(hipStreamSynchronize(s));
+    bool all = true;
+    for (int b = 0; b < nb; b++) all = all && hst[4 * b + 1];
+    return all;
+  };
+  uint32_t tag = 0;
+  for (int bk = 0; bk < nblk; bk++) {
+    const int b0 = bk * CB, b1 = std::min(maxsteps, b0 + CB);
+    bool ran = false;
+    if (coop) {
+      int i0 = b0, i1 = b1, wm = wg_max;
+      ++tag;  // one tag per launch: every exchange word is written once
+      void* args[] = {(void*)&d_desc, &d_chunk, &g,        &ws,  &piv,
+                      &state,        &i0,      &i1,       &wm,  &xch,
+                      &d_lt_off,     &lt,      &tag,      &tol, &fault};
+      const hipError_t e = hipLaunchCooperativeKernel(
+          reinterpret_cast<const void*>(&pchol_coop), dim3(nchunk),
+          dim3(256), args, 0, s);
+      if (e == hipSuccess) {
+        ran = true;
+      } else {
+        (void)hipGetLastError();  // clear the launch error
+        note_coop_refused();
+        coop = false;
+      }
+    }
+    bool stop = false;
+    if (!ran) {
+      for (int i = b0; i < b1; i++) {
+        hipLaunchKernelGGL(pchol_step, grid, dim3(256), 0, s, d_desc, g, ws,
+                           piv, state, i, b0, i & 1, tol);
+        // stop early once every box has met its rank
+        if ((i & 63) == 63 && all_done()) {
+          stop = true;
+          break;
         }
       }
-      if (!lib)
-        launch_gemm<double>(d_g + base, nb, maxn, maxn, ws, ws, g, s, 0, 1);
+    } else {
+      stop = all_done();
     }
-    if ((i & 63) == 63) {  // stop early once every box has met its rank
-      KI_HIP(hipMemcpyAsync(hst.data(), state, sizeof(int) * 4 * nb,
-                            hipMemcpyDeviceToHost, s));
-      KI_HIP(hipStreamSynchronize(s));
-      bool all = true;
-      for (int b = 0; b < nb; b++) all = all && hst[4 * b + 1];
-      if (all) break;
+    if (stop || b1 >= maxsteps) break;
+    // S -= L_blk L_blk^T is a plain GEMM per box: for a few large boxes the
+    // library's tuned kernel, else one batched launch
+    const size_t base = (size_t)bk * nb;
+    bool lib = nb <= 128 && maxn >= 512, any = false;
+    for (int b = 0; lib && b < nb; b++) {
+      const GemmDesc& q = gd[base + b];
+      if (q.m == 0) continue;
+      if (library_gemm<double>(q, ws, ws, g, s)) {
+        any = true;
+      } else {
+        KI_CHECK(!any, 3, "rocBLAS GEMM failed after earlier successes");
+        lib = false;
+      }
     }
+    if (!lib)
+      launch_gemm<double>(d_g + base, nb, maxn, maxn, ws, ws, g, s, 0, 1);
   }
-  (void)i;
+  if (xch != nullptr) {
+    int hf = 0;
+    KI_HIP(hipMemcpyAsync(&hf, fault, sizeof(int), hipMemcpyDeviceToHost, s));
+    KI_HIP(hipStreamSynchronize(s));
+    KI_HIP(hipFreeAsync(xch, s));
+    KI_HIP(hipFreeAsync(lt, s));
+    KI_HIP(hipFreeAsync(d_lt_off, s));
+    KI_HIP(hipFreeAsync(d_chunk, s));
+    KI_CHECK(hf == 0, 3, "pivoted Cholesky exchange timed out");
+  }
   hipLaunchKernelGGL(pchol_order, dim3(nb), dim3(64), 0, s, d_desc,
                      (const double*)ws, piv, (const int*)state, rank);
   const int t64 = (maxn + 63) / 64;

@@ -195,7 +195,12 @@ void pchol_small(const CholDesc* d_desc, int ndesc, double* g, double* ws,
                  int* piv, int* rank, double tol, hipStream_t s);
 void pchol_large(const std::vector<CholDesc>& h, const CholDesc* d_desc,
                  double* g, double* ws, int* piv, int* state, int* rank,
-                 double tol, hipStream_t s);
+                 double tol, bool coop, hipStream_t s);
+// true once the runtime has refused a cooperative launch in this process
+// (another process may hold the cooperative-launch resources); the
+// multi-workgroup kernels then use their per-step launch forms
+bool coop_refused();
+void note_coop_refused();
 
 // T = R11^{-1} R12 in place (rows 0..k-1, cols k..n-1 of the QR output).
 template <typename T>

@@ -1,0 +1,120 @@
+// Tagged exchange words for multi-workgroup kernels of one cooperative
+// launch.
+//
+// A posted word is a 64-bit relaxed agent-scope atomic that holds 32 data
+// bits and a 32-bit tag.  A reader spins on each word it needs until the
+// tag is the one it expects; per-location coherence of the atomic alone
+// makes the data bits valid, so no barrier, counter or cache-wide release /
+// acquire fence (an L2 write-back and invalidate per exchange) is needed.
+// The host zeroes a buffer before its first launch and never reuses a tag
+// while the buffer lives (tag 0 is never used).  Every wait is bounded: a
+// workgroup that gives up sets *fault, the other waiters see it and stop,
+// and the host turns it into an error instead of a hang.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ki {
+
+constexpr long long kExchangeSpins = 1ll << 24;
+
+__device__ __forceinline__ void tag_put(uint64_t* p, uint32_t bits,
+                                        uint32_t tag) {
+  __hip_atomic_store(p, ((uint64_t)tag << 32) | bits, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t tag_get(uint64_t* p, uint32_t tag,
+                                            int* fault) {
+  for (long long spins = 0;; spins++) {
+    const uint64_t w =
+        __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((uint32_t)(w >> 32) == tag) return (uint32_t)w;
+    if ((spins & 255) == 255) {
+      if (__hip_atomic_load(fault, __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_AGENT) != 0)
+        return 0;
+      if (spins > kExchangeSpins) {
+        __hip_atomic_store(fault, 1, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        return 0;
+      }
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+// N consecutive tagged words read together: the N loads are in flight at
+// once and the wait ends when every tag matches (one round trip, not N).
+template <int N>
+__device__ __forceinline__ void tag_get_seq(uint64_t* p, uint32_t tag,
+                                            int* fault, uint32_t (&out)[N]) {
+  for (long long spins = 0;; spins++) {
+    uint64_t w[N];
+#pragma unroll
+    for (int k = 0; k < N; k++)
+      w[k] = __hip_atomic_load(p + k, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < N; k++) ok = ok && (uint32_t)(w[k] >> 32) == tag;
+    if (ok) {
+#pragma unroll
+      for (int k = 0; k < N; k++) out[k] = (uint32_t)w[k];
+      return;
+    }
+    if ((spins & 255) == 255) {
+      if (__hip_atomic_load(fault, __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_AGENT) != 0)
+        break;
+      if (spins > kExchangeSpins) {
+        __hip_atomic_store(fault, 1, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+#pragma unroll
+  for (int k = 0; k < N; k++) out[k] = 0;
+}
+
+// a value as tagged words: one for float, two (low, high) for double
+template <typename T>
+struct Tagged;
+template <>
+struct Tagged<float> {
+  static constexpr int W = 1;
+  __device__ static void put(uint64_t* p, float v, uint32_t tag) {
+    tag_put(p, __float_as_uint(v), tag);
+  }
+  __device__ static float get(uint64_t* p, uint32_t tag, int* fault) {
+    return __uint_as_float(tag_get(p, tag, fault));
+  }
+  __device__ static float from(const uint32_t* w) {
+    return __uint_as_float(w[0]);
+  }
+};
+template <>
+struct Tagged<double> {
+  static constexpr int W = 2;
+  __device__ static void put(uint64_t* p, double v, uint32_t tag) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    tag_put(p, (uint32_t)b, tag);
+    tag_put(p + 1, (uint32_t)(b >> 32), tag);
+  }
+  __device__ static double get(uint64_t* p, uint32_t tag, int* fault) {
+    uint32_t w[2];
+    tag_get_seq<2>(p, tag, fault, w);
+    return __longlong_as_double(
+        (long long)((uint64_t)w[0] | ((uint64_t)w[1] << 32)));
+  }
+  __device__ static double from(const uint32_t* w) {
+    return __longlong_as_double(
+        (long long)((uint64_t)w[0] | ((uint64_t)w[1] << 32)));
+  }
+};
+
+}  // namespace ki

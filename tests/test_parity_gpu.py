@@ -293,6 +293,32 @@ def test_gram_and_householder_ids_agree(name):
 
 
 @pytest.mark.parametrize("precision", [64, 32])
+def test_cooperative_and_per_step_cholesky_agree(precision):
+    """The large-box pivoted Cholesky of the Gram ID, one block of pivot
+    steps per cooperative launch (chol_coop=1) or one launch per step
+    (chol_coop=0): same expressions in the same order, so the same ranks,
+    the same skeletons and bit-identical solutions."""
+    b, pde, kw, leaf, prm, ctx, okern, otree = setup("stokes3d_ellipsoid")
+    rng = np.random.default_rng(12)
+    rhs = rng.normal(size=ctx.ndof)
+    out = []
+    for coop in (0, 1):
+        ctx.skeletonize(ki.default_params(precision=precision, id_method=2,
+                                          chol_coop=coop, **kw))
+        sk = {(l, i): (tuple(ctx.box_list(l, i, "skel")),
+                       len(ctx.box_list(l, i, "redundant")))
+              for l in range(ctx.num_levels())
+              for i in range(ctx.level_size(l))
+              if ctx.box(l, i)["compressed"]}
+        out.append((sk, ctx.solve(rhs)))
+    (sk0, x0), (sk1, x1) = out
+    # boxes above 128 DOFs take the multi-workgroup path
+    assert max(len(a) + r for a, r in sk0.values()) > 128
+    assert sk0 == sk1
+    assert np.array_equal(x0, x1)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
 @pytest.mark.parametrize("knob", [{"lu_wide_rows": 256},
                                   {"lu_coop_rows": 64}])
 def test_multi_workgroup_lu_panels_match_one_workgroup_panels(knob,
