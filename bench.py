@@ -52,7 +52,22 @@ BOUNDS = {"gram_kernel": ("mfma", PEAK_FP64),
           "cpqr_wg": ("hbm", PEAK_HBM)}
 
 
-def roofline(clock):
+def recorded_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated), or
+    None when no summary covers the pair."""
+    for f in sorted((ROOT / "profiles").glob("*_traffic.md")):
+        kv = {}
+        for line in f.read_text().splitlines():
+            k, sep, v = line.partition(": ")
+            if sep and k in ("workload", "kernel", "traffic_bytes_per_launch"):
+                kv[k] = v.strip()
+        if kv.get("workload") == workload and kv.get("kernel") == kernel:
+            return float(kv["traffic_bytes_per_launch"])
+    return None
+
+
+def roofline(clock, workload):
     """Roofline entry of the single kernel with the largest event time."""
     cand = [k for k in clock if k["name"] in BOUNDS and k["launches"] > 0]
     if not cand:
@@ -64,10 +79,12 @@ def roofline(clock):
     scale = 1e12 if bound == "mfma" else 1e9
     return {"bound": bound, "achieved": achieved / scale, "peak": peak / scale,
             "unit": "TFLOP/s" if bound == "mfma" else "GB/s",
-            "frac": achieved / peak, "traffic": None, "kernel": k["name"],
-            "launches": k["launches"],
+            "frac": achieved / peak,
+            "traffic": recorded_traffic(k["name"], workload),
+            "kernel": k["name"], "launches": k["launches"],
             "avg_launch_us": 1e6 * k["seconds"] / k["launches"],
-            "algorithmic_per_launch": work / k["launches"]}
+            "algorithmic_per_launch": work / k["launches"],
+            "algorithmic_bytes_per_launch": k["bytes"] / k["launches"]}
 
 
 def train_predictor(ki, G, ctx, index, scale):
@@ -246,7 +263,7 @@ def main():
 
     # roofline of the dominant single kernel of the timed factorization,
     # from the library's HIP-event clocks on its own stream
-    roof = roofline(clocks[-1])
+    roof = roofline(clocks[-1], pb.name)
 
     cpu = None
     if not args.no_cpu_baseline:
