@@ -66,7 +66,11 @@ typedef struct ki_params {
                                runs a block of pivot steps per cooperative
                                launch when all its workgroups fit; 0: one
                                launch per pivot step                          */
-  int reserved[4];
+  int lu_block;             /* 1: the two-level LU factors 128-wide blocks
+                               whole, one cooperative launch each, when
+                               their workgroups fit; 0: 256-wide outer
+                               panels of 32-wide inner panels only            */
+  int reserved[3];
 } ki_params;
 
 void ki_params_default(ki_params* p);

@@ -56,7 +56,8 @@ class KiParams(C.Structure):
         ("lu_coop_rows", C.c_int),
         ("solve_graph", C.c_int),
         ("chol_coop", C.c_int),
-        ("reserved", C.c_int * 4),
+        ("lu_block", C.c_int),
+        ("reserved", C.c_int * 3),
     ]
 
 

@@ -1498,6 +1498,7 @@ void ki_params_default(ki_params* p) {
   p->fp32_tol_floor = 1e-6;
   p->solve_graph = 1;
   p->chol_coop = 1;
+  p->lu_block = 1;
 }
 
 const char* ki_last_error(void) { return ki::g_err.c_str(); }
@@ -1685,9 +1686,11 @@ int ki_skeletonize(ki_ctx* ctx, const ki_params* params) {
     c.factored = false;
     c.prec = c.prm.precision;
     struct Thresholds {  // this call's LU thresholds, defaults afterwards
-      Thresholds(int w, int cr) { ki::set_lu_thresholds(w, cr); }
-      ~Thresholds() { ki::set_lu_thresholds(0, 0); }
-    } th(c.prm.lu_wide_rows, c.prm.lu_coop_rows);
+      Thresholds(int w, int cr, bool blk) {
+        ki::set_lu_thresholds(w, cr, blk);
+      }
+      ~Thresholds() { ki::set_lu_thresholds(0, 0, true); }
+    } th(c.prm.lu_wide_rows, c.prm.lu_coop_rows, c.prm.lu_block != 0);
     if (c.prec == 32)
       ki::skeletonize_t<float>(c);
     else

@@ -434,50 +434,57 @@ lu_pstep(T* __restrict__ A, int ld, int m, int j0, int jb,
 constexpr int PR = 224;  // panel rows per workgroup (LDS: PR x LB values)
 
 // Exchange words of one matrix: per column step, wg_max candidate records
-// (row index, |value|, LB row entries) followed by row c's LB entries.
-template <typename T>
+// (row index, |value|, PW row entries) followed by row c's PW entries.
+template <typename T, int PW>
 struct PanelXch {
   static constexpr int W = Tagged<T>::W;
-  static constexpr int kRec = 1 + W + LB * W;
+  static constexpr int kRec = 1 + W + PW * W;
   __host__ __device__ static int64_t step_words(int wg_max) {
-    return (int64_t)wg_max * kRec + LB * W;
+    return (int64_t)wg_max * kRec + PW * W;
   }
   __host__ __device__ static int64_t words(int wg_max) {
-    return LB * step_words(wg_max);
+    return PW * step_words(wg_max);
   }
 };
 
-template <typename T>
+// Panel shapes: PW columns per panel, PRW rows per workgroup held in LDS.
+//   LB x PR: the 32-column inner panels (57 KB of LDS in FP64)
+//   BW x BR: the 128-column block panels of the top-level LU, which factor
+//            a whole 128-wide block in one launch (132 KB of LDS in FP64,
+//            one workgroup per CU)
+constexpr int BW = 128, BR = 128;
+
+template <typename T, int PW, int PRW>
 __global__ void __launch_bounds__(256)
 lu_coop_panel(const MatDesc* __restrict__ desc, T* __restrict__ a_all,
               int* __restrict__ piv_all, int j0, int jend, int wg_max,
               uint64_t* __restrict__ xch_all, uint32_t tag0,
               int* __restrict__ fault) {
   typedef Tagged<T> TW;
-  typedef PanelXch<T> X;
+  typedef PanelXch<T, PW> X;
   // matrix blockIdx.y of the batch; its panel is columns [j0, j0 + jb) with
   // jb limited by min(m, n) and by jend (the enclosing outer panel's end)
   const int box = blockIdx.y;
   const MatDesc dm = desc[box];
   const int lim = min(jend, min(dm.m, dm.n));
   if (j0 >= lim) return;
-  const int jb = min(LB, lim - j0);
+  const int jb = min(PW, lim - j0);
   const int m = dm.m, ld = dm.ld;
   T* A = a_all + dm.off;
   int* piv = piv_all + dm.piv_off;
   const int mr = m - j0;
-  const int nwg = (mr + PR - 1) / PR, b = blockIdx.x, tid = threadIdx.x;
+  const int nwg = (mr + PRW - 1) / PRW, b = blockIdx.x, tid = threadIdx.x;
   if (b >= nwg) return;  // this matrix needs fewer workgroups
   uint64_t* xch = xch_all + (int64_t)box * X::words(wg_max);
-  const int r0 = b * PR;
-  const int nr = min(PR, mr - r0);
-  __shared__ T P[LB][PR + 1];  // P[q][i] = A[j0 + r0 + i, j0 + q]
-  __shared__ T prow[LB];
+  const int r0 = b * PRW;
+  const int nr = min(PRW, mr - r0);
+  __shared__ T P[PW][PRW + 1];  // P[q][i] = A[j0 + r0 + i, j0 + q]
+  __shared__ T prow[PW];
   __shared__ T rv[4];
   __shared__ int ri[4];
-  __shared__ int s_p;
-  for (int e = tid; e < LB * PR; e += 256) {
-    const int i = e % PR, q = e / PR;
+  __shared__ int s_p, s_w;
+  for (int e = tid; e < PW * PRW; e += 256) {
+    const int i = e % PRW, q = e / PRW;
     P[q][i] = (i < nr && q < jb) ? A[(j0 + r0 + i) + (int64_t)ld * (j0 + q)]
                                  : (T)0;
   }
@@ -557,26 +564,29 @@ lu_coop_panel(const MatDesc* __restrict__ desc, T* __restrict__ a_all,
       int p = (ii == INT_MAX || ii < c || ii >= mr || !(bb > 0) || ww < 0)
                   ? c
                   : ii;
-      if (tid < jb)
-        prow[tid] =
-            p == c ? TW::get(rowc + X::W * tid, tag, fault)
-                   : TW::get(step + (int64_t)ww * X::kRec + 1 + X::W +
-                                 X::W * tid,
-                             tag, fault);
       if (tid == 0) {
         s_p = p;
+        s_w = ww;
         if (b == 0) piv[j0 + c] = j0 + p;
       }
     }
     __syncthreads();
     const int p = s_p;
-    // interchange rows c and p (row c's entries: the posted copy)
-    if (p != c && tid < jb) {
-      if (p >= r0 && p < r0 + nr)
-        P[tid][p - r0] = c >= r0 && c < r0 + nr
-                             ? P[tid][c - r0]
-                             : TW::get(rowc + X::W * tid, tag, fault);
-      if (c >= r0 && c < r0 + nr) P[tid][c - r0] = prow[tid];
+    if (tid < jb) {
+      // the pivot row, then the interchange of rows c and p (row c's
+      // entries: the posted copy); thread tid handles column tid
+      const T pr = p == c ? TW::get(rowc + X::W * tid, tag, fault)
+                          : TW::get(step + (int64_t)s_w * X::kRec + 1 + X::W +
+                                        X::W * tid,
+                                    tag, fault);
+      prow[tid] = pr;
+      if (p != c) {
+        if (p >= r0 && p < r0 + nr)
+          P[tid][p - r0] = c >= r0 && c < r0 + nr
+                               ? P[tid][c - r0]
+                               : TW::get(rowc + X::W * tid, tag, fault);
+        if (c >= r0 && c < r0 + nr) P[tid][c - r0] = pr;
+      }
     }
     __syncthreads();
     // scale and rank-1 update of local rows below c
@@ -588,8 +598,8 @@ lu_coop_panel(const MatDesc* __restrict__ desc, T* __restrict__ a_all,
     }
     __syncthreads();
   }
-  for (int e = tid; e < LB * PR; e += 256) {
-    const int i = e % PR, q = e / PR;
+  for (int e = tid; e < PW * PRW; e += 256) {
+    const int i = e % PRW, q = e / PRW;
     if (i < nr && q < jb) A[(j0 + r0 + i) + (int64_t)ld * (j0 + q)] = P[q][i];
   }
 }
@@ -941,12 +951,21 @@ static void trsm_sweeps(const std::vector<MatDesc>& h, MatDesc* d_desc,
                         const T* a, const int* piv, T* b, const int* modes,
                         int nmodes, int swap, hipStream_t s);
 
-// LU of one large matrix with two levels of blocking: 256-wide outer panels
-// whose trailing update is one K = 256 GEMM, each factored as 32-wide inner
-// panels (multi-workgroup column steps, interchanges and TRSM confined to the
-// outer panel, K = 32 GEMM inside it).  Same pivots as the one-level form.
+// LU of one large matrix, right-looking.  First form: 128-wide block
+// panels, each factored whole by one cooperative launch (128 rows of the
+// block per workgroup in LDS), then the row interchanges on the other
+// columns, U12 = L11^{-1} A12 and one K = 128 trailing GEMM.  When the
+// block panel's workgroups cannot all be resident (taller matrices), or
+// the runtime refuses a cooperative launch, the remaining columns go to the
+// two-level form: 256-wide outer panels whose trailing update is one
+// K = 256 GEMM, each factored as 32-wide inner panels (multi-workgroup
+// column steps, interchanges and TRSM confined to the outer panel, K = 32
+// GEMM inside it).  Both give partial pivoting in LAPACK getf2 order.
 template <typename T>
 static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s);
+template <typename T>
+static void lu_two_level(const MatDesc& x, T* a, int* piv_all, int start,
+                         hipStream_t s);
 
 template <typename T>
 bool library_gemm(const GemmDesc& q, const T* a, const T* b, T* c,
@@ -981,6 +1000,7 @@ template bool library_gemm<double>(const GemmDesc&, const double*,
 
 namespace {
 thread_local int g_lu_wide_rows = 0, g_lu_coop_rows = 0;
+thread_local bool g_lu_no_block = false;
 // Set when the runtime refused a cooperative launch (for instance while
 // another process on the GPU holds the cooperative-launch resources): the
 // LUs of this process then use the per-column / one-workgroup panels.
@@ -990,14 +1010,16 @@ bool g_coop_refused = false;
 bool coop_refused() { return g_coop_refused; }
 void note_coop_refused() { g_coop_refused = true; }
 
-void set_lu_thresholds(int wide_rows, int coop_rows) {
+void set_lu_thresholds(int wide_rows, int coop_rows, bool block) {
   g_lu_wide_rows = wide_rows;
   g_lu_coop_rows = coop_rows;
+  g_lu_no_block = !block;
 }
 
 // Scratch and launches of the cooperative panel kernel for a batch of
-// matrices that advance panel by panel together.
-template <typename T>
+// matrices that advance panel by panel together (PW-column panels, PRW rows
+// per workgroup).
+template <typename T, int PW = LB, int PRW = PR>
 struct CoopPanels {
   int nb = 0, wg_max = 0, npanels = 0, panel_idx = 0;
   MatDesc* d_desc = nullptr;
@@ -1016,7 +1038,7 @@ struct CoopPanels {
       maxm = std::max(maxm, x.m);
       maxmn = std::max(maxmn, std::min(x.m, x.n));
     }
-    wg_max = (maxm + PR - 1) / PR;
+    wg_max = (maxm + PRW - 1) / PRW;
     int dev = 0, ncu = 0, can = 0, per_cu = 0;
     KI_HIP(hipGetDevice(&dev));
     KI_HIP(hipDeviceGetAttribute(&can, hipDeviceAttributeCooperativeLaunch,
@@ -1024,13 +1046,13 @@ struct CoopPanels {
     KI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
                                  dev));
     KI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void*>(&lu_coop_panel<T>), 256, 0));
+        &per_cu, reinterpret_cast<const void*>(&lu_coop_panel<T, PW, PRW>), 256, 0));
     if (!can || (int64_t)per_cu * ncu < (int64_t)wg_max * nb) return false;
-    npanels = (maxmn + LB - 1) / LB;
-    // tags are 1 + panel * LB + column: unique while they fit in 32 bits
-    KI_CHECK((int64_t)npanels * LB < (1ll << 31), 2,
+    npanels = (maxmn + PW - 1) / PW;
+    // tags are 1 + panel * PW + column: unique while they fit in 32 bits
+    KI_CHECK((int64_t)npanels * PW < (1ll << 31), 2,
              "too many LU panels for the exchange tags");
-    const size_t words = (size_t)PanelXch<T>::words(wg_max) * nb + 1;
+    const size_t words = (size_t)PanelXch<T, PW>::words(wg_max) * nb + 1;
     KI_HIP(hipMallocAsync((void**)&d_desc, sizeof(MatDesc) * nb, s));
     KI_HIP(hipMemcpyAsync(d_desc, h.data(), sizeof(MatDesc) * nb,
                           hipMemcpyHostToDevice, s));
@@ -1041,16 +1063,16 @@ struct CoopPanels {
     return true;
   }
 
-  // Factor columns [j0, min(j0 + LB, jend)) of every matrix.  Returns false,
+  // Factor columns [j0, min(j0 + PW, jend)) of every matrix.  Returns false,
   // with nothing launched, when the runtime refuses the cooperative launch;
   // the caller then factors this and the remaining panels another way.
   bool panel(T* a, int* piv_all, int j0, int jend, hipStream_t s) {
     KI_CHECK(panel_idx < npanels, 2, "cooperative panel count exceeded");
-    uint32_t tag0 = 1u + (uint32_t)panel_idx * LB;
+    uint32_t tag0 = 1u + (uint32_t)panel_idx * PW;
     void* args[] = {&d_desc, &a,   &piv_all, &j0,   &jend,
                     &wg_max, &xch, &tag0,    &fault};
     const hipError_t e = hipLaunchCooperativeKernel(
-        reinterpret_cast<const void*>(&lu_coop_panel<T>), dim3(wg_max, nb),
+        reinterpret_cast<const void*>(&lu_coop_panel<T, PW, PRW>), dim3(wg_max, nb),
         dim3(256), args, 0, s);
     if (e != hipSuccess) {
       (void)hipGetLastError();  // clear the launch error
@@ -1073,6 +1095,75 @@ struct CoopPanels {
 
 template <typename T>
 static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
+  const int m = x.m, n = x.n, ld = x.ld, mn = std::min(m, n);
+  T* A = a + x.off;
+  int* piv = piv_all + x.piv_off;
+  int J0 = 0;
+  CoopPanels<T, BW, BR> bp;
+  if (!g_lu_no_block && bp.init(std::vector<MatDesc>{x}, s)) {
+    MatDesc* d_m = nullptr;
+    GemmDesc* d_g = nullptr;
+    KI_HIP(hipMallocAsync((void**)&d_m, sizeof(MatDesc), s));
+    KI_HIP(hipMallocAsync((void**)&d_g, sizeof(GemmDesc), s));
+    auto cols = [](int c) { return dim3((std::max(c, 1) + 255) / 256); };
+    for (; J0 < mn; J0 += BW) {
+      const int JB = std::min(BW, mn - J0);
+      if (!bp.panel(a, piv_all, J0, J0 + JB, s)) break;
+      if (J0 > 0)
+        hipLaunchKernelGGL(swap_cols_range<T>, cols(J0), dim3(256), 0, s, A,
+                           ld, (const int*)piv, J0, J0 + JB, 0, J0);
+      if (n <= J0 + JB) continue;
+      hipLaunchKernelGGL(swap_cols_range<T>, cols(n - J0 - JB), dim3(256), 0,
+                         s, A, ld, (const int*)piv, J0, J0 + JB, J0 + JB, n);
+      // U12 = L11^{-1} A12 (unit lower JB x JB), blocked
+      MatDesc md{};
+      md.m = md.n = JB;
+      md.off = x.off + J0 + (int64_t)ld * J0;
+      md.ld = ld;
+      md.b_off = x.off + J0 + (int64_t)ld * (J0 + JB);
+      md.ldb = ld;
+      md.nrhs = n - J0 - JB;
+      std::vector<MatDesc> hv{md};
+      KI_HIP(hipMemcpyAsync(d_m, &hv[0], sizeof(MatDesc),
+                            hipMemcpyHostToDevice, s));
+      const int mode0[1] = {0};
+      trsm_sweeps<T>(hv, d_m, a, nullptr, a, mode0, 1, 0, s);
+      if (m > J0 + JB) {
+        // A22 -= L21 U12: a plain K = JB GEMM, the library's when available
+        const GemmDesc q{m - J0 - JB,
+                         n - J0 - JB,
+                         JB,
+                         x.off + (J0 + JB) + (int64_t)ld * J0,
+                         x.off + J0 + (int64_t)ld * (J0 + JB),
+                         x.off + (J0 + JB) + (int64_t)ld * (J0 + JB),
+                         ld,
+                         ld,
+                         ld,
+                         0,
+                         0,
+                         -1.0,
+                         1.0};
+        if (!library_gemm<T>(q, a, a, a, s)) {
+          KI_HIP(hipMemcpyAsync(d_g, &q, sizeof(GemmDesc),
+                                hipMemcpyHostToDevice, s));
+          launch_gemm<T>(d_g, 1, q.m, q.n, a, a, a, s, 0, 0);
+          KI_HIP(hipStreamSynchronize(s));  // q dies with this scope
+        }
+      }
+      // (trsm_sweeps synchronizes before returning, so hv may die here)
+    }
+    KI_HIP(hipGetLastError());
+    KI_HIP(hipStreamSynchronize(s));
+    KI_HIP(hipFreeAsync(d_m, s));
+    KI_HIP(hipFreeAsync(d_g, s));
+    bp.finish(s);
+  }
+  if (J0 < mn) lu_two_level<T>(x, a, piv_all, J0, s);
+}
+
+template <typename T>
+static void lu_two_level(const MatDesc& x, T* a, int* piv_all, int start,
+                         hipStream_t s) {
   constexpr int NB = 256;
   const int m = x.m, n = x.n, ld = x.ld, mn = std::min(m, n);
   T* A = a + x.off;
@@ -1085,7 +1176,7 @@ static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
     gd.push_back(g);
   };
   // pass 1: descriptors in execution order
-  for (int J0 = 0; J0 < mn; J0 += NB) {
+  for (int J0 = start; J0 < mn; J0 += NB) {
     const int JB = std::min(NB, mn - J0);
     for (int j0 = J0; j0 < J0 + JB; j0 += LB) {
       const int jb = std::min(LB, J0 + JB - j0);
@@ -1110,7 +1201,7 @@ static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
   bool coop = cp_live;
   size_t gi = 0;
   auto cols = [](int c) { return dim3((std::max(c, 1) + 255) / 256); };
-  for (int J0 = 0; J0 < mn; J0 += NB) {
+  for (int J0 = start; J0 < mn; J0 += NB) {
     const int JB = std::min(NB, mn - J0);
     for (int j0 = J0; j0 < J0 + JB; j0 += LB) {
       const int jb = std::min(LB, J0 + JB - j0);

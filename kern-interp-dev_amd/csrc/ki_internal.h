@@ -256,7 +256,7 @@ void batched_lu(const std::vector<MatDesc>& h, MatDesc* d_desc, T* a,
 // default): a single matrix taller than wide_rows takes the two-level
 // blocked form; batches taller than coop_rows factor each panel over many
 // workgroups.
-void set_lu_thresholds(int wide_rows, int coop_rows);
+void set_lu_thresholds(int wide_rows, int coop_rows, bool block);
 
 // X = U^{-1} B for upper-triangular U (no interchanges), blocked.
 template <typename T>

@@ -320,13 +320,15 @@ def test_cooperative_and_per_step_cholesky_agree(precision):
 
 @pytest.mark.parametrize("precision", [64, 32])
 @pytest.mark.parametrize("knob", [{"lu_wide_rows": 256},
+                                  {"lu_wide_rows": 256, "lu_block": 0},
                                   {"lu_coop_rows": 64}])
 def test_multi_workgroup_lu_panels_match_one_workgroup_panels(knob,
                                                               precision):
     """The multi-workgroup (cooperative) LU panels -- the top-level LU's
-    two-level form (lu_wide_rows) and the batched decouple LU's
-    (lu_coop_rows), forced onto this small case -- give the same solution as
-    the one-workgroup panels: 1e-10 (FP64) / 1e-4 (FP32) relative."""
+    128-wide block panels and its two-level form (lu_wide_rows, with and
+    without lu_block) and the batched decouple LU's (lu_coop_rows), forced
+    onto this small case -- give the same solution as the one-workgroup
+    panels: 1e-10 (FP64) / 1e-4 (FP32) relative."""
     b, pde, kw, leaf, prm, ctx, okern, otree = setup("stokes3d_ellipsoid")
     rng = np.random.default_rng(8)
     rhs = rng.normal(size=ctx.ndof)
