@@ -656,7 +656,13 @@ __global__ void panel_copy_back(T* __restrict__ A, int ld, int m, int j0,
 }
 
 // Apply the panel's row interchanges to the columns outside the panel, then
-// A12 <- L11^{-1} A12 for the columns to the right (one thread per column).
+// A12 <- L11^{-1} A12 for the columns to the right.  Each column is held by
+// 32 lanes of a wave (lane t: row j0 + t), loaded and stored with one
+// coalesced 128-byte access; interchanges inside the panel rows are lane
+// shuffles, those with rows below it one global exchange by the owning
+// lane, in LAPACK laswp order; the unit-lower solve broadcasts row q's value
+// to the lanes below it, q ascending (the same update order per entry as a
+// column-at-a-time solve).  Eight columns per 256-thread workgroup.
 template <typename T>
 __global__ void __launch_bounds__(256)
 lu_swap_trsm(const MatDesc* __restrict__ desc, T* __restrict__ a_all,
@@ -665,28 +671,40 @@ lu_swap_trsm(const MatDesc* __restrict__ desc, T* __restrict__ a_all,
   const int m = d.m, n = d.n;
   if (j0 >= min(m, n)) return;
   const int jb = min(LB, min(m, n) - j0);
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = threadIdx.x & 31;
+  const int col = blockIdx.x * 8 + (threadIdx.x >> 5);
   const int ncols_out = n - jb;
+  // whole 32-lane groups leave together, so the shuffles below only mix
+  // lanes of live columns
   if (col >= ncols_out) return;
   const int c = col < j0 ? col : col + jb;
   T* A = a_all + d.off;
   const int* piv = piv_all + d.piv_off;
   T* cc = A + (int64_t)d.ld * c;
+  const int base = threadIdx.x & 32;  // first lane of this column's group
+  T v = t < jb ? cc[j0 + t] : (T)0;
   for (int q = 0; q < jb; q++) {
     const int r = j0 + q, p = piv[r];
-    if (p != r) {
-      const T t = cc[r];
-      cc[r] = cc[p];
-      cc[p] = t;
+    if (p == r) continue;
+    if (p < j0 + jb) {  // both rows in the panel: exchange lanes q, p - j0
+      const int pq = p - j0;
+      const T vq = __shfl(v, base + q, 64), vp = __shfl(v, base + pq, 64);
+      if (t == q) v = vp;
+      if (t == pq) v = vq;
+    } else if (t == q) {
+      const T w = cc[p];
+      cc[p] = v;
+      v = w;
     }
   }
   if (c >= j0 + jb) {
+    const T* L = A + j0 + (int64_t)d.ld * j0;  // L11, unit lower
     for (int q = 0; q < jb; q++) {
-      const T x = cc[j0 + q];
-      for (int t = q + 1; t < jb; t++)
-        cc[j0 + t] -= A[(j0 + t) + (int64_t)d.ld * (j0 + q)] * x;
+      const T x = __shfl(v, base + q, 64);
+      if (t > q && t < jb) v -= L[t + (int64_t)d.ld * q] * x;
     }
   }
+  if (t < jb) cc[j0 + t] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -1328,7 +1346,7 @@ void batched_lu(const std::vector<MatDesc>& h, MatDesc* d_desc, T* a, int* piv,
     if (!coop)
       hipLaunchKernelGGL(lu_panel<T>, dim3(nb), dim3(256), 0, s, d_desc, a,
                          piv, j0);
-    hipLaunchKernelGGL(lu_swap_trsm<T>, dim3((maxn + 255) / 256, nb),
+    hipLaunchKernelGGL(lu_swap_trsm<T>, dim3((maxn + 7) / 8, nb),
                        dim3(256), 0, s, d_desc, a, piv, j0);
     if (mm[p] > 0 && nn[p] > 0)
       launch_gemm<T>(d_g + (size_t)p * nb, nb, mm[p], nn[p], a, a, a, s, 0,
