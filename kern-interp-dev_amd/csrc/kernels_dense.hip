@@ -604,6 +604,66 @@ lu_coop_panel(const MatDesc* __restrict__ desc, T* __restrict__ a_all,
   }
 }
 
+// U12 <- L11^{-1} U12 for one block row of a single matrix: L11 =
+// A[j0:j0+jb, j0:j0+jb] unit lower (jb <= BW), U12 = A[j0:j0+jb, c0:c1).
+// Each right-hand-side column is held by 32 lanes of a wave, lane t0 owning
+// rows t0, t0 + 32, t0 + 64, t0 + 96 in registers (coalesced 128-byte
+// column loads and stores).  The forward substitution runs q ascending
+// without barriers: row q's value is broadcast by a lane shuffle and the
+// rows below q take one FMA each; the entries of L11's column q are loads
+// that every column's lanes share through L1/L2.  Eight columns per
+// 256-thread workgroup.
+template <typename T>
+__global__ void __launch_bounds__(256)
+trsm_block_row(T* __restrict__ A, int ld, int j0, int jb, int c0, int c1) {
+  constexpr int NBLK = BW / 32;
+  const int lane = threadIdx.x & 63, t0 = lane & 31, base = lane & 32;
+  const int c = c0 + (blockIdx.x * 4 + (threadIdx.x >> 6)) * 2 + (lane >> 5);
+  // whole 32-lane groups leave together: shuffles stay inside a group
+  if (c >= c1) return;
+  T* col = A + j0 + (int64_t)ld * c;
+  const T* L = A + j0 + (int64_t)ld * j0;
+  T b[NBLK];
+#pragma unroll
+  for (int i = 0; i < NBLK; i++) {
+    const int t = t0 + 32 * i;
+    b[i] = t < jb ? col[t] : (T)0;
+  }
+  // L11 loads are unconditional (indices clamped into the block) and the
+  // updates selected, so that the unrolled steps' loads can be in flight
+  // together instead of each waiting behind a branch
+  constexpr int U = 8;  // steps whose loads are issued together
+#pragma unroll
+  for (int blk = 0; blk < NBLK; blk++) {
+    const int qb = 32 * blk;
+    if (qb >= jb) break;
+    for (int qq0 = 0; qq0 < 32; qq0 += U) {
+      T lv[U][NBLK];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const T* Lq = L + (int64_t)ld * min(qb + qq0 + u, jb - 1);
+#pragma unroll
+        for (int i = blk; i < NBLK; i++) lv[u][i] = Lq[min(t0 + 32 * i, jb - 1)];
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int q = qb + qq0 + u;
+        const T x = __shfl(b[blk], base + qq0 + u, 64);
+#pragma unroll
+        for (int i = blk; i < NBLK; i++) {
+          const int t = t0 + 32 * i;
+          if (t > q && t < jb) b[i] -= lv[u][i] * x;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NBLK; i++) {
+    const int t = t0 + 32 * i;
+    if (t < jb) col[t] = b[i];
+  }
+}
+
 // Row interchanges piv[r0..r1) applied, in order, to columns [clo, chi) of a
 // single matrix (one thread per column).
 template <typename T>
@@ -1119,9 +1179,7 @@ static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
   int J0 = 0;
   CoopPanels<T, BW, BR> bp;
   if (!g_lu_no_block && bp.init(std::vector<MatDesc>{x}, s)) {
-    MatDesc* d_m = nullptr;
     GemmDesc* d_g = nullptr;
-    KI_HIP(hipMallocAsync((void**)&d_m, sizeof(MatDesc), s));
     KI_HIP(hipMallocAsync((void**)&d_g, sizeof(GemmDesc), s));
     auto cols = [](int c) { return dim3((std::max(c, 1) + 255) / 256); };
     for (; J0 < mn; J0 += BW) {
@@ -1133,19 +1191,9 @@ static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
       if (n <= J0 + JB) continue;
       hipLaunchKernelGGL(swap_cols_range<T>, cols(n - J0 - JB), dim3(256), 0,
                          s, A, ld, (const int*)piv, J0, J0 + JB, J0 + JB, n);
-      // U12 = L11^{-1} A12 (unit lower JB x JB), blocked
-      MatDesc md{};
-      md.m = md.n = JB;
-      md.off = x.off + J0 + (int64_t)ld * J0;
-      md.ld = ld;
-      md.b_off = x.off + J0 + (int64_t)ld * (J0 + JB);
-      md.ldb = ld;
-      md.nrhs = n - J0 - JB;
-      std::vector<MatDesc> hv{md};
-      KI_HIP(hipMemcpyAsync(d_m, &hv[0], sizeof(MatDesc),
-                            hipMemcpyHostToDevice, s));
-      const int mode0[1] = {0};
-      trsm_sweeps<T>(hv, d_m, a, nullptr, a, mode0, 1, 0, s);
+      // U12 = L11^{-1} A12 (unit lower JB x JB)
+      hipLaunchKernelGGL(trsm_block_row<T>, dim3((n - J0 - JB + 7) / 8),
+                         dim3(256), 0, s, A, ld, J0, JB, J0 + JB, n);
       if (m > J0 + JB) {
         // A22 -= L21 U12: a plain K = JB GEMM, the library's when available
         const GemmDesc q{m - J0 - JB,
@@ -1168,11 +1216,9 @@ static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
           KI_HIP(hipStreamSynchronize(s));  // q dies with this scope
         }
       }
-      // (trsm_sweeps synchronizes before returning, so hv may die here)
     }
     KI_HIP(hipGetLastError());
     KI_HIP(hipStreamSynchronize(s));
-    KI_HIP(hipFreeAsync(d_m, s));
     KI_HIP(hipFreeAsync(d_g, s));
     bp.finish(s);
   }
