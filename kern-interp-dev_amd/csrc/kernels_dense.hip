@@ -768,75 +768,116 @@ lu_swap_trsm(const MatDesc* __restrict__ desc, T* __restrict__ a_all,
 }
 
 // ---------------------------------------------------------------------------
-// Triangular solves with the LU factors, blocked by LB rows:
+// Triangular solves with the LU factors, blocked by BW = 128 rows:
 //   mode 0: L (unit lower) forward      mode 1: U (upper) backward
 //   mode 2: U^T (lower) forward         mode 3: L^T (unit upper) backward
-// The diagonal-block kernel stages the LB x LB triangle and a block of TC
-// right-hand-side columns in LDS (coalesced column reads), solves each column
-// with one thread, and writes the block back; GEMM updates the other rows.
+// The diagonal-block kernel holds each right-hand-side column in 32 lanes
+// of a wave (lane t0: rows t0 + 32 i of the block, in registers; coalesced
+// column loads and stores) and substitutes row by row without barriers:
+// row q's value (divided by the diagonal in the non-unit modes) is
+// broadcast by a lane shuffle and the rows on the far side of q take one
+// FMA each, in the same per-entry order as a column-at-a-time solve.  The
+// factor entries are loads served by L1/L2, issued eight rows at a time
+// with clamped indices so they are in flight together.  Eight columns per
+// 256-thread workgroup; a GEMM then updates the rows outside the block.
 // ---------------------------------------------------------------------------
-constexpr int TC = 128;  // right-hand-side columns per workgroup
-
-template <typename T>
-__global__ void __launch_bounds__(TC)
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256)
 trsm_diag(const MatDesc* __restrict__ desc, const T* __restrict__ a_all,
-          T* __restrict__ b_all, int blk, int mode) {
+          T* __restrict__ b_all, int blk) {
+  constexpr bool FWD = MODE == 0 || MODE == 2;
+  constexpr bool TR = MODE >= 2;  // the operator is the factor transposed
+  constexpr bool UNIT = MODE == 0 || MODE == 3;
+  constexpr int NB4 = BW / 32, U = 8;
   const MatDesc d = desc[blockIdx.y];
   const int n = d.n;
-  const int nblk = (n + LB - 1) / LB;
+  const int nblk = (n + BW - 1) / BW;
   if (blk >= nblk || d.nrhs <= 0) return;
-  const int c0 = blockIdx.x * TC;
-  if (c0 >= d.nrhs) return;
-  const int nc = min(TC, d.nrhs - c0);
-  // forward modes walk blocks top-down, backward modes bottom-up
-  const int b = (mode == 0 || mode == 2) ? blk : nblk - 1 - blk;
-  const int r0 = b * LB, rb = min(LB, n - r0);
-  const T* A = a_all + d.off;
-  const int ld = d.ld;
-  // D[t][q] = A(r0 + t, r0 + q); transposed modes read D[q][t]
-  __shared__ T D[LB][LB + 1];
-  __shared__ T X[LB][TC + 1];  // X[t][j] = B(r0 + t, c0 + j)
-  for (int e = threadIdx.x; e < LB * LB; e += TC) {
-    const int t = e % LB, q = e / LB;  // identity outside the rb x rb block
-    D[t][q] = (t < rb && q < rb) ? A[(r0 + t) + (int64_t)ld * (r0 + q)]
-                                 : (T)(t == q ? 1 : 0);
+  const int lane = threadIdx.x & 63, t0 = lane & 31, base = lane & 32;
+  const int c = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 2 + (lane >> 5);
+  // whole 32-lane groups leave together: shuffles stay inside a group
+  if (c >= d.nrhs) return;
+  const int bb = FWD ? blk : nblk - 1 - blk;
+  const int r0 = bb * BW, rb = min(BW, n - r0);
+  const int64_t ld = d.ld;
+  const T* D = a_all + d.off + r0 + ld * r0;  // the factor's diagonal block
+  // entry (t, q) of the triangular operator, indices clamped into the block
+  auto op = [&](int t, int q) {
+    t = min(t, rb - 1);
+    q = min(q, rb - 1);
+    return TR ? D[q + ld * t] : D[t + ld * q];
+  };
+  T* col = b_all + d.b_off + r0 + (int64_t)d.ldb * c;
+  T v[NB4];
+#pragma unroll
+  for (int i = 0; i < NB4; i++) {
+    const int t = t0 + 32 * i;
+    v[i] = t < rb ? col[t] : (T)0;
   }
-  T* Bp = b_all + d.b_off + r0 + (int64_t)d.ldb * c0;
-  for (int e = threadIdx.x; e < LB * TC; e += TC) {
-    const int t = e % LB, j = e / LB;
-    X[t][j] = (t < rb && j < nc) ? Bp[t + (int64_t)d.ldb * j] : (T)0;
-  }
-  __syncthreads();
-  const int j = threadIdx.x;
-  if (j < nc) {
-    if (mode == 0) {  // unit lower
-      for (int q = 0; q < rb; q++) {
-        const T v = X[q][j];
-        for (int t = q + 1; t < rb; t++) X[t][j] -= D[t][q] * v;
+  if (FWD) {
+#pragma unroll
+    for (int g = 0; g < NB4; g++) {
+      if (32 * g >= rb) break;
+      for (int qq0 = 0; qq0 < 32; qq0 += U) {
+        T lv[U][NB4], dq[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int q = 32 * g + qq0 + u;
+          if (!UNIT) dq[u] = op(q, q);
+#pragma unroll
+          for (int i = g; i < NB4; i++) lv[u][i] = op(t0 + 32 * i, q);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int q = 32 * g + qq0 + u;
+          T x = __shfl(v[g], base + qq0 + u, 64);
+          if (!UNIT) {
+            x = x / dq[u];
+            if (t0 == qq0 + u && q < rb) v[g] = x;
+          }
+#pragma unroll
+          for (int i = g; i < NB4; i++) {
+            const int t = t0 + 32 * i;
+            if (t > q && t < rb) v[i] -= lv[u][i] * x;
+          }
+        }
       }
-    } else if (mode == 1) {  // upper, non-unit
-      for (int q = rb - 1; q >= 0; q--) {
-        const T v = X[q][j] / D[q][q];
-        X[q][j] = v;
-        for (int t = 0; t < q; t++) X[t][j] -= D[t][q] * v;
-      }
-    } else if (mode == 2) {  // U^T: lower, non-unit, (U^T)(t,q) = D[q][t]
-      for (int q = 0; q < rb; q++) {
-        const T v = X[q][j] / D[q][q];
-        X[q][j] = v;
-        for (int t = q + 1; t < rb; t++) X[t][j] -= D[q][t] * v;
-      }
-    } else {  // L^T: unit upper, (L^T)(t,q) = D[q][t]
-      for (int q = rb - 1; q >= 0; q--) {
-        const T v = X[q][j];
-        for (int t = 0; t < q; t++) X[t][j] -= D[q][t] * v;
+    }
+  } else {
+#pragma unroll
+    for (int g = NB4 - 1; g >= 0; g--) {
+      if (32 * g >= rb) continue;
+      for (int qq0 = 32 - U; qq0 >= 0; qq0 -= U) {
+        T lv[U][NB4], dq[U];
+#pragma unroll
+        for (int u = U - 1; u >= 0; u--) {
+          const int q = 32 * g + qq0 + u;
+          if (!UNIT) dq[u] = op(q, q);
+#pragma unroll
+          for (int i = 0; i <= g; i++) lv[u][i] = op(t0 + 32 * i, q);
+        }
+#pragma unroll
+        for (int u = U - 1; u >= 0; u--) {
+          const int q = 32 * g + qq0 + u;
+          if (q >= rb) continue;
+          T x = __shfl(v[g], base + qq0 + u, 64);
+          if (!UNIT) {
+            x = x / dq[u];
+            if (t0 == qq0 + u) v[g] = x;
+          }
+#pragma unroll
+          for (int i = 0; i <= g; i++) {
+            const int t = t0 + 32 * i;
+            if (t < q) v[i] -= lv[u][i] * x;
+          }
+        }
       }
     }
   }
-  __syncthreads();
-  for (int e = threadIdx.x; e < LB * TC; e += TC) {
-    const int t = e % LB, jj = e / LB;
-    if (t < rb && jj < nc) Bp[t + (int64_t)d.ldb * jj] = X[t][jj];
+#pragma unroll
+  for (int i = 0; i < NB4; i++) {
+    const int t = t0 + 32 * i;
+    if (t < rb) col[t] = v[i];
   }
 }
 
@@ -1420,7 +1461,7 @@ static void trsm_sweeps(const std::vector<MatDesc>& h, MatDesc* d_desc,
     maxr = std::max(maxr, x.nrhs);
   }
   if (maxr == 0) return;
-  const int nblk = (maxn + LB - 1) / LB;
+  const int nblk = (maxn + BW - 1) / BW;
   // GEMM descriptors for every sweep and block, uploaded once.
   std::vector<GemmDesc> gd((size_t)nmodes * nblk * nb);
   std::vector<int> mm(nmodes * nblk, 0), nn(nmodes * nblk, 0);
@@ -1431,12 +1472,12 @@ static void trsm_sweeps(const std::vector<MatDesc>& h, MatDesc* d_desc,
       const int slot = mi * nblk + blk;
       for (int q = 0; q < nb; q++) {
         const MatDesc& x = h[q];
-        const int xb = (x.n + LB - 1) / LB;
+        const int xb = (x.n + BW - 1) / BW;
         GemmDesc& g = gd[(size_t)slot * nb + q];
         g = GemmDesc{0, 0, 0, 0, 0, 0, 1, 1, 1, 0, 0, -1.0, 1.0};
         if (blk >= xb || x.nrhs == 0) continue;
         const int bb = fwd ? blk : xb - 1 - blk;
-        const int r0 = bb * LB, rb = std::min(LB, x.n - r0);
+        const int r0 = bb * BW, rb = std::min(BW, x.n - r0);
         g.n = x.nrhs;
         g.k = rb;
         g.b_off = x.b_off + r0;
@@ -1483,8 +1524,24 @@ static void trsm_sweeps(const std::vector<MatDesc>& h, MatDesc* d_desc,
   for (int mi = 0; mi < nmodes; mi++) {
     for (int blk = 0; blk < nblk; blk++) {
       const int slot = mi * nblk + blk;
-      hipLaunchKernelGGL(trsm_diag<T>, dim3((maxr + TC - 1) / TC, nb),
-                         dim3(TC), 0, s, d_desc, a, b, blk, modes[mi]);
+      const dim3 tg((maxr + 7) / 8, nb);
+      switch (modes[mi]) {
+        case 0:
+          hipLaunchKernelGGL((trsm_diag<T, 0>), tg, dim3(256), 0, s, d_desc,
+                             a, b, blk);
+          break;
+        case 1:
+          hipLaunchKernelGGL((trsm_diag<T, 1>), tg, dim3(256), 0, s, d_desc,
+                             a, b, blk);
+          break;
+        case 2:
+          hipLaunchKernelGGL((trsm_diag<T, 2>), tg, dim3(256), 0, s, d_desc,
+                             a, b, blk);
+          break;
+        default:
+          hipLaunchKernelGGL((trsm_diag<T, 3>), tg, dim3(256), 0, s, d_desc,
+                             a, b, blk);
+      }
       if (mm[slot] > 0 && nn[slot] > 0)
         launch_gemm<T>(d_g + (size_t)slot * nb, nb, mm[slot], nn[slot], a, b,
                        b, s, modes[mi] >= 2 ? 1 : 0, 0);
