@@ -778,8 +778,12 @@ lu_swap_trsm(const MatDesc* __restrict__ desc, T* __restrict__ a_all,
 // broadcast by a lane shuffle and the rows on the far side of q take one
 // FMA each, in the same per-entry order as a column-at-a-time solve.  The
 // factor entries are loads served by L1/L2, issued eight rows at a time
-// with clamped indices so they are in flight together.  Eight columns per
-// 256-thread workgroup; a GEMM then updates the rows outside the block.
+// with clamped indices so they are in flight together.  The transposed
+// modes would read the factor across its rows (one cache line per lane),
+// so they first stage the diagonal block in LDS with coalesced column
+// reads and serve 32 columns per workgroup from it; the others take eight
+// columns per 256-thread workgroup.  A GEMM then updates the rows outside
+// the block.
 // ---------------------------------------------------------------------------
 template <typename T, int MODE>
 __global__ void __launch_bounds__(256)
@@ -788,96 +792,113 @@ trsm_diag(const MatDesc* __restrict__ desc, const T* __restrict__ a_all,
   constexpr bool FWD = MODE == 0 || MODE == 2;
   constexpr bool TR = MODE >= 2;  // the operator is the factor transposed
   constexpr bool UNIT = MODE == 0 || MODE == 3;
-  constexpr int NB4 = BW / 32, U = 8;
+  // rows whose factor entries are fetched together (LDS reads need few)
+  constexpr int NB4 = BW / 32, U = TR ? 2 : 8;
+  constexpr int ROUNDS = TR ? 4 : 1;  // groups of 8 columns per workgroup
   const MatDesc d = desc[blockIdx.y];
   const int n = d.n;
   const int nblk = (n + BW - 1) / BW;
   if (blk >= nblk || d.nrhs <= 0) return;
-  const int lane = threadIdx.x & 63, t0 = lane & 31, base = lane & 32;
-  const int c = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 2 + (lane >> 5);
-  // whole 32-lane groups leave together: shuffles stay inside a group
-  if (c >= d.nrhs) return;
+  const int c00 = blockIdx.x * 8 * ROUNDS;
+  if (c00 >= d.nrhs) return;
   const int bb = FWD ? blk : nblk - 1 - blk;
   const int r0 = bb * BW, rb = min(BW, n - r0);
   const int64_t ld = d.ld;
   const T* D = a_all + d.off + r0 + ld * r0;  // the factor's diagonal block
+  __shared__ T S[TR ? BW : 1][TR ? BW + 1 : 1];  // S[r][c] = D(r, c)
+  if (TR) {
+    for (int e = threadIdx.x; e < BW * BW; e += 256) {
+      const int r = e % BW, cc = e / BW;
+      S[r][cc] = (r < rb && cc < rb) ? D[r + ld * cc] : (T)0;
+    }
+    __syncthreads();
+  }
   // entry (t, q) of the triangular operator, indices clamped into the block
   auto op = [&](int t, int q) {
     t = min(t, rb - 1);
     q = min(q, rb - 1);
-    return TR ? D[q + ld * t] : D[t + ld * q];
+    return TR ? S[q][t] : D[t + ld * q];
   };
-  T* col = b_all + d.b_off + r0 + (int64_t)d.ldb * c;
-  T v[NB4];
+  const int lane = threadIdx.x & 63, t0 = lane & 31, base = lane & 32;
+#pragma unroll 1
+  for (int rnd = 0; rnd < ROUNDS; rnd++) {
+    const int c = c00 + rnd * 8 + (threadIdx.x >> 5);
+    // whole 32-lane groups skip together: shuffles stay inside a group
+    if (c >= d.nrhs) break;
+    T* col = b_all + d.b_off + r0 + (int64_t)d.ldb * c;
+    T v[NB4];
 #pragma unroll
-  for (int i = 0; i < NB4; i++) {
-    const int t = t0 + 32 * i;
-    v[i] = t < rb ? col[t] : (T)0;
-  }
-  if (FWD) {
+    for (int i = 0; i < NB4; i++) {
+      const int t = t0 + 32 * i;
+      v[i] = t < rb ? col[t] : (T)0;
+    }
+    if (FWD) {
 #pragma unroll
-    for (int g = 0; g < NB4; g++) {
-      if (32 * g >= rb) break;
-      for (int qq0 = 0; qq0 < 32; qq0 += U) {
-        T lv[U][NB4], dq[U];
+      for (int g = 0; g < NB4; g++) {
+        if (32 * g >= rb) break;
+#pragma unroll 1
+        for (int qq0 = 0; qq0 < 32; qq0 += U) {
+          T lv[U][NB4], dq[U];
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-          const int q = 32 * g + qq0 + u;
-          if (!UNIT) dq[u] = op(q, q);
+          for (int u = 0; u < U; u++) {
+            const int q = 32 * g + qq0 + u;
+            if (!UNIT) dq[u] = op(q, q);
 #pragma unroll
-          for (int i = g; i < NB4; i++) lv[u][i] = op(t0 + 32 * i, q);
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-          const int q = 32 * g + qq0 + u;
-          T x = __shfl(v[g], base + qq0 + u, 64);
-          if (!UNIT) {
-            x = x / dq[u];
-            if (t0 == qq0 + u && q < rb) v[g] = x;
+            for (int i = g; i < NB4; i++) lv[u][i] = op(t0 + 32 * i, q);
           }
 #pragma unroll
-          for (int i = g; i < NB4; i++) {
-            const int t = t0 + 32 * i;
-            if (t > q && t < rb) v[i] -= lv[u][i] * x;
+          for (int u = 0; u < U; u++) {
+            const int q = 32 * g + qq0 + u;
+            T x = __shfl(v[g], base + qq0 + u, 64);
+            if (!UNIT) {
+              x = x / dq[u];
+              if (t0 == qq0 + u && q < rb) v[g] = x;
+            }
+#pragma unroll
+            for (int i = g; i < NB4; i++) {
+              const int t = t0 + 32 * i;
+              if (t > q && t < rb) v[i] -= lv[u][i] * x;
+            }
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int g = NB4 - 1; g >= 0; g--) {
+        if (32 * g >= rb) continue;
+#pragma unroll 1
+        for (int qq0 = 32 - U; qq0 >= 0; qq0 -= U) {
+          T lv[U][NB4], dq[U];
+#pragma unroll
+          for (int u = U - 1; u >= 0; u--) {
+            const int q = 32 * g + qq0 + u;
+            if (!UNIT) dq[u] = op(q, q);
+#pragma unroll
+            for (int i = 0; i <= g; i++) lv[u][i] = op(t0 + 32 * i, q);
+          }
+#pragma unroll
+          for (int u = U - 1; u >= 0; u--) {
+            const int q = 32 * g + qq0 + u;
+            if (q >= rb) continue;
+            T x = __shfl(v[g], base + qq0 + u, 64);
+            if (!UNIT) {
+              x = x / dq[u];
+              if (t0 == qq0 + u) v[g] = x;
+            }
+#pragma unroll
+            for (int i = 0; i <= g; i++) {
+              const int t = t0 + 32 * i;
+              if (t < q) v[i] -= lv[u][i] * x;
+            }
           }
         }
       }
     }
-  } else {
 #pragma unroll
-    for (int g = NB4 - 1; g >= 0; g--) {
-      if (32 * g >= rb) continue;
-      for (int qq0 = 32 - U; qq0 >= 0; qq0 -= U) {
-        T lv[U][NB4], dq[U];
-#pragma unroll
-        for (int u = U - 1; u >= 0; u--) {
-          const int q = 32 * g + qq0 + u;
-          if (!UNIT) dq[u] = op(q, q);
-#pragma unroll
-          for (int i = 0; i <= g; i++) lv[u][i] = op(t0 + 32 * i, q);
-        }
-#pragma unroll
-        for (int u = U - 1; u >= 0; u--) {
-          const int q = 32 * g + qq0 + u;
-          if (q >= rb) continue;
-          T x = __shfl(v[g], base + qq0 + u, 64);
-          if (!UNIT) {
-            x = x / dq[u];
-            if (t0 == qq0 + u) v[g] = x;
-          }
-#pragma unroll
-          for (int i = 0; i <= g; i++) {
-            const int t = t0 + 32 * i;
-            if (t < q) v[i] -= lv[u][i] * x;
-          }
-        }
-      }
+    for (int i = 0; i < NB4; i++) {
+      const int t = t0 + 32 * i;
+      if (t < rb) col[t] = v[i];
     }
-  }
-#pragma unroll
-  for (int i = 0; i < NB4; i++) {
-    const int t = t0 + 32 * i;
-    if (t < rb) col[t] = v[i];
   }
 }
 
@@ -1524,7 +1545,9 @@ static void trsm_sweeps(const std::vector<MatDesc>& h, MatDesc* d_desc,
   for (int mi = 0; mi < nmodes; mi++) {
     for (int blk = 0; blk < nblk; blk++) {
       const int slot = mi * nblk + blk;
-      const dim3 tg((maxr + 7) / 8, nb);
+      // the transposed modes serve 32 columns per workgroup (8 otherwise)
+      const int cpw = modes[mi] >= 2 ? 32 : 8;
+      const dim3 tg((maxr + cpw - 1) / cpw, nb);
       switch (modes[mi]) {
         case 0:
           hipLaunchKernelGGL((trsm_diag<T, 0>), tg, dim3(256), 0, s, d_desc,
