@@ -70,7 +70,10 @@ typedef struct ki_params {
                                whole, one cooperative launch each, when
                                their workgroups fit; 0: 256-wide outer
                                panels of 32-wide inner panels only            */
-  int reserved[3];
+  int solve_wave;           /* 1: the factor apply's blocked LU solves run
+                               as one wavefront launch per triangle; 0: one
+                               launch pair per 64-row block                  */
+  int reserved[2];
 } ki_params;
 
 void ki_params_default(ki_params* p);

@@ -57,7 +57,8 @@ class KiParams(C.Structure):
         ("solve_graph", C.c_int),
         ("chol_coop", C.c_int),
         ("lu_block", C.c_int),
-        ("reserved", C.c_int * 3),
+        ("solve_wave", C.c_int),
+        ("reserved", C.c_int * 2),
     ]
 
 

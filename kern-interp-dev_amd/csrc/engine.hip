@@ -116,6 +116,9 @@ struct Ctx {
   DevBuf facptrs;                   // device array of per-level pointers
   // block-diagonal solve x[r] = X_rr^{-1} x[r] of every compressed box
   DevBuf dg_desc, dg_gidx, dg_sidx, dg_work;
+  // wavefront solves: tagged words, then the generation and fault words
+  DevBuf dg_xch, top_xch;
+  int64_t dg_xw = 0, top_xw = 0;
   int dg_n = 0, dg_max_r = 0;
   int64_t dg_total = 0;
   // top-level solve plan (same kernels, one FP64 block)
@@ -1208,6 +1211,9 @@ void skeletonize_t(Ctx& c) {
     upload(c.dg_gidx, dgi, s);
     upload(c.dg_sidx, dsi, s);
     c.dg_work.ensure(sizeof(T) * c.dg_total);
+    c.dg_xw = diag_wave_words(c.dg_total, c.dg_n, sizeof(T));
+    c.dg_xch.ensure(sizeof(uint64_t) * (c.dg_xw + 1));
+    KI_HIP(hipMemsetAsync(c.dg_xch.p, 0, sizeof(uint64_t) * (c.dg_xw + 1), s));
   }
   if (nt > 0) {
     std::vector<int> tp(nt);
@@ -1224,6 +1230,10 @@ void skeletonize_t(Ctx& c) {
     std::vector<double*> tptr{c.top.as<double>()};
     upload(c.top_ptr, tptr, s);
     c.top_vec.ensure(sizeof(double) * nt);
+    c.top_xw = diag_wave_words(nt, 1, sizeof(double));
+    c.top_xch.ensure(sizeof(uint64_t) * (c.top_xw + 1));
+    KI_HIP(hipMemsetAsync(c.top_xch.p, 0, sizeof(uint64_t) * (c.top_xw + 1),
+                          s));
   }
   KI_HIP(hipStreamSynchronize(s));
   st.device_bytes = dev_bytes + c.top.bytes;
@@ -1253,17 +1263,36 @@ void solve_launches(Ctx& c, const double* b_in, double* x_out) {
     phase(LD, 0);
     phase(LD, 1);
   }
-  // x[r] = X_rr^{-1} x[r] for every compressed box
-  launch_diag_blocked<T, T>(c.dg_desc.as<DiagDesc>(), c.dg_n, c.dg_max_r,
-                            c.dg_gidx.as<int>(), c.dg_sidx.as<int>(),
-                            c.dg_total, fb, xt, c.dg_work.as<T>(), s);
-  // top level, FP64
+  // x[r] = X_rr^{-1} x[r] for every compressed box, then the top level in
+  // FP64; the last two words of each exchange buffer are its generation
+  // counter and fault flag
   const int nt = c.allskel.size();
-  if (nt > 0)
-    launch_diag_blocked<T, double>(
-        c.top_ddesc.as<DiagDesc>(), 1, nt, c.top_gidx.as<int>(),
-        c.top_sidx.as<int>(), nt, c.top_ptr.as<double*>(), xt,
-        c.top_vec.as<double>(), s);
+  if (c.prm.solve_wave) {
+    uint64_t* dx = c.dg_xch.as<uint64_t>();
+    if (c.dg_n > 0)
+      launch_diag_wave<T, T>(
+          c.dg_desc.as<DiagDesc>(), c.dg_n, c.dg_max_r, c.dg_gidx.as<int>(),
+          c.dg_sidx.as<int>(), c.dg_total, fb, xt, c.dg_work.as<T>(), dx,
+          reinterpret_cast<unsigned*>(dx + c.dg_xw),
+          reinterpret_cast<int*>(dx + c.dg_xw) + 1, s);
+    uint64_t* tx = c.top_xch.as<uint64_t>();
+    if (nt > 0)
+      launch_diag_wave<T, double>(
+          c.top_ddesc.as<DiagDesc>(), 1, nt, c.top_gidx.as<int>(),
+          c.top_sidx.as<int>(), nt, c.top_ptr.as<double*>(), xt,
+          c.top_vec.as<double>(), tx,
+          reinterpret_cast<unsigned*>(tx + c.top_xw),
+          reinterpret_cast<int*>(tx + c.top_xw) + 1, s);
+  } else {
+    launch_diag_blocked<T, T>(c.dg_desc.as<DiagDesc>(), c.dg_n, c.dg_max_r,
+                              c.dg_gidx.as<int>(), c.dg_sidx.as<int>(),
+                              c.dg_total, fb, xt, c.dg_work.as<T>(), s);
+    if (nt > 0)
+      launch_diag_blocked<T, double>(
+          c.top_ddesc.as<DiagDesc>(), 1, nt, c.top_gidx.as<int>(),
+          c.top_sidx.as<int>(), nt, c.top_ptr.as<double*>(), xt,
+          c.top_vec.as<double>(), s);
+  }
   // down-sweeps: x[r] -= U x[s]; x[s] -= T x[r]  (levels 0 .. L-1)
   for (int l = 0; l < L; l++) {
     LevelDev& LD = *c.lev[l];
@@ -1499,6 +1528,7 @@ void ki_params_default(ki_params* p) {
   p->solve_graph = 1;
   p->chol_coop = 1;
   p->lu_block = 1;
+  p->solve_wave = 1;
 }
 
 const char* ki_last_error(void) { return ki::g_err.c_str(); }
@@ -1699,6 +1729,27 @@ int ki_skeletonize(ki_ctx* ctx, const ki_params* params) {
   });
 }
 
+namespace {
+// A wavefront solve that timed out on the device leaves its fault word set
+// (the result of that solve is then not valid): report it once.
+void check_solve_faults(Ctx& c) {
+  if (!c.prm.solve_wave) return;
+  for (ki::DevBuf* b : {&c.dg_xch, &c.top_xch}) {
+    const int64_t xw = b == &c.dg_xch ? c.dg_xw : c.top_xw;
+    if (b->p == nullptr || xw == 0) continue;
+    int f = 0;
+    int* fp = reinterpret_cast<int*>(b->as<uint64_t>() + xw) + 1;
+    KI_HIP(hipMemcpyAsync(&f, fp, sizeof(int), hipMemcpyDeviceToHost, c.s));
+    KI_HIP(hipStreamSynchronize(c.s));
+    if (f != 0) {
+      KI_HIP(hipMemsetAsync(fp, 0, sizeof(int), c.s));
+      KI_HIP(hipStreamSynchronize(c.s));
+      ki::fail(KI_E_DEVICE, "wavefront triangular solve timed out");
+    }
+  }
+}
+}  // namespace
+
 int ki_solve(ki_ctx* ctx, const double* b, double* x) {
   return guard([&] {
     Ctx& c = ctx->c;
@@ -1713,6 +1764,7 @@ int ki_solve(ki_ctx* ctx, const double* b, double* x) {
     KI_HIP(hipMemcpyAsync(x, c.d_x.p, sizeof(double) * c.ndof,
                           hipMemcpyDeviceToHost, c.s));
     KI_HIP(hipStreamSynchronize(c.s));
+    check_solve_faults(c);
   });
 }
 
@@ -1758,6 +1810,7 @@ int ki_gmres_resident(ki_ctx* ctx, double tol, int maxit, int restart,
     ki::solve_dev(c, c.d_b.as<double>(), x);  // x0 = M^{-1} b
     ki::gmres_dev(c, c.d_b.as<double>(), x, tol, maxit, restart, iters,
                   relres);
+    check_solve_faults(c);
   });
 }
 

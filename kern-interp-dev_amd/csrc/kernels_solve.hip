@@ -18,6 +18,7 @@
 #include <cmath>
 
 #include "ki_internal.h"
+#include "xch.h"
 
 namespace ki {
 
@@ -207,6 +208,91 @@ trsv_update_kernel(const DiagDesc* __restrict__ desc,
   T p = 0;
   for (int q = 0; q < rb; q++) p += X[row + (int64_t)r * (r0 + q)] * ws[q];
   w[row] -= p;
+}
+
+// ---------------------------------------------------------------------------
+// The same blocked LU solves in one launch per triangle (wavefront form).
+// Workgroup (k, box) owns the box's k-th block of DB rows in the order of
+// the sweep (top-down for L, bottom-up for U).  It takes the solved values
+// of the earlier blocks as their owners post them, accumulating its rows'
+// products with those blocks' columns while the factor entries for the
+// next block are already loading, then solves its diagonal block (one
+// wave, lane shuffles) and posts its own values.  Values travel as tagged
+// words (xch.h); the tag is 2 gen + upper, with gen bumped on the device
+// by wave_bump before each solve, so replays of a captured launch sequence
+// never see a previous solve's words.  A workgroup waits only for lower-
+// numbered workgroups of the same box, which in-order dispatch has already
+// placed; the waits are bounded all the same (a timeout sets *fault).
+// ---------------------------------------------------------------------------
+__global__ void wave_bump(unsigned* __restrict__ gen) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *gen += 1u;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+trsv_wave_kernel(const DiagDesc* __restrict__ desc, T* const* __restrict__ fac,
+                 T* __restrict__ w_all, uint64_t* __restrict__ xch_all,
+                 const unsigned* __restrict__ gen, int upper,
+                 int* __restrict__ fault) {
+  typedef Tagged<T> TW;
+  constexpr int W = TW::W, CPT = DB / 4;  // columns per thread and block
+  const DiagDesc d = desc[blockIdx.y];
+  const int r = d.r, nb = (r + DB - 1) / DB;
+  const int k = blockIdx.x;
+  if (k >= nb) return;
+  const uint32_t tag = 2u * *gen + (uint32_t)upper;
+  const int bb = upper ? nb - 1 - k : k;
+  const int r0 = bb * DB, rb = min(DB, r - r0);
+  const T* X = fac[d.buf] + d.offX;
+  T* w = w_all + d.w_off;
+  // this box's words: DB per block, at W (w_off + DB box) (disjoint ranges)
+  uint64_t* xch = xch_all + (int64_t)W * (d.w_off + (int64_t)DB * blockIdx.y);
+  __shared__ T D[DB][DB + 1];  // D[t][q] = X(r0 + t, r0 + q)
+  __shared__ T xs[2][DB];
+  __shared__ T red[4][DB];
+  const int t = threadIdx.x & 63, cq = threadIdx.x >> 6;
+  for (int q = cq; q < rb; q += 4)
+    D[t][q] = t < rb ? X[(r0 + t) + (int64_t)r * (r0 + q)] : (T)0;
+  T acc = 0;
+  for (int jj = 0; jj < k; jj++) {
+    const int jb = upper ? nb - 1 - jj : jj;
+    const int c0 = jb * DB, cb = min(DB, r - c0);
+    T xv[CPT];
+#pragma unroll
+    for (int i = 0; i < CPT; i++) {
+      const int c = cq + 4 * i;
+      xv[i] = (t < rb && c < cb) ? X[(r0 + t) + (int64_t)r * (c0 + c)] : (T)0;
+    }
+    if (threadIdx.x < DB)
+      xs[jj & 1][t] =
+          t < cb ? TW::get(xch + (int64_t)W * (c0 + t), tag, fault) : (T)0;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < CPT; i++) acc += xv[i] * xs[jj & 1][cq + 4 * i];
+  }
+  red[cq][t] = acc;
+  __syncthreads();
+  if (cq != 0) return;
+  T x = t < rb ? w[r0 + t] - ((red[0][t] + red[1][t]) + (red[2][t] + red[3][t]))
+               : (T)0;
+  if (!upper) {
+    for (int q = 0; q < rb; q++) {
+      const T v = __shfl(x, q, 64);
+      if (t > q) x -= D[t][q] * v;
+    }
+  } else {
+    for (int q = rb - 1; q >= 0; q--) {
+      const T v = __shfl(x, q, 64) / D[q][q];
+      if (t == q)
+        x = v;
+      else if (t < q)
+        x -= D[t][q] * v;
+    }
+  }
+  if (t < rb) {
+    w[r0 + t] = x;
+    TW::put(xch + (int64_t)W * (r0 + t), x, tag);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -459,6 +545,45 @@ void launch_diag_blocked(const DiagDesc* d, int nd, int max_r,
                      (const TW*)w, sidx, total, x);
   KI_HIP(hipGetLastError());
 }
+
+template <typename TX, typename TW>
+void launch_diag_wave(const DiagDesc* d, int nd, int max_r, const int* gidx,
+                      const int* sidx, int64_t total, TW* const* facbufs,
+                      TX* x, TW* w, uint64_t* xch, unsigned* gen, int* fault,
+                      hipStream_t s) {
+  if (nd == 0 || total == 0) return;
+  const int gx = (int)std::min<int64_t>((total + 255) / 256, 8192);
+  hipLaunchKernelGGL((diag_gather_kernel<TX, TW>), dim3(gx), dim3(256), 0, s,
+                     (const TX*)x, gidx, total, w);
+  hipLaunchKernelGGL(wave_bump, dim3(1), dim3(64), 0, s, gen);
+  const int nb = (max_r + DB - 1) / DB;
+  for (int upper = 0; upper < 2; upper++)
+    hipLaunchKernelGGL(trsv_wave_kernel<TW>, dim3(nb, nd), dim3(256), 0, s, d,
+                       facbufs, w, xch, (const unsigned*)gen, upper, fault);
+  hipLaunchKernelGGL((diag_scatter_kernel<TW, TX>), dim3(gx), dim3(256), 0, s,
+                     (const TW*)w, sidx, total, x);
+  KI_HIP(hipGetLastError());
+}
+
+int64_t diag_wave_words(int64_t total, int nd, int elem_bytes) {
+  return (elem_bytes / 4) * (total + (int64_t)DB * nd);
+}
+
+template void launch_diag_wave<float, float>(const DiagDesc*, int, int,
+                                             const int*, const int*, int64_t,
+                                             float* const*, float*, float*,
+                                             uint64_t*, unsigned*, int*,
+                                             hipStream_t);
+template void launch_diag_wave<double, double>(const DiagDesc*, int, int,
+                                               const int*, const int*, int64_t,
+                                               double* const*, double*,
+                                               double*, uint64_t*, unsigned*,
+                                               int*, hipStream_t);
+template void launch_diag_wave<float, double>(const DiagDesc*, int, int,
+                                              const int*, const int*, int64_t,
+                                              double* const*, float*, double*,
+                                              uint64_t*, unsigned*, int*,
+                                              hipStream_t);
 
 template void launch_sweep_phase<float>(const SweepDesc*, const SweepItem*,
                                         int, const int*, float* const*,

@@ -313,6 +313,16 @@ template <typename TX, typename TW>
 void launch_diag_blocked(const DiagDesc* d, int nd, int max_r,
                          const int* gidx, const int* sidx, int64_t total,
                          TW* const* facbufs, TX* x, TW* w, hipStream_t s);
+// The same solves in one launch per triangle (wavefront over the row
+// blocks, values passed through tagged words): xch holds
+// diag_wave_words(total, nd, sizeof(TW)) zeroed words, gen and fault one
+// zeroed word each, all kept for the plan's lifetime.
+template <typename TX, typename TW>
+void launch_diag_wave(const DiagDesc* d, int nd, int max_r, const int* gidx,
+                      const int* sidx, int64_t total, TW* const* facbufs,
+                      TX* x, TW* w, uint64_t* xch, unsigned* gen, int* fault,
+                      hipStream_t s);
+int64_t diag_wave_words(int64_t total, int nd, int elem_bytes);
 
 // ---------------------------------------------------------------------------
 // Learned rank predictor (12-64-64-1 perceptron, flat FP32 parameters).

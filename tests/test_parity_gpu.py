@@ -411,6 +411,30 @@ def test_solve_graph_replays_match_eager_solves(precision):
             assert np.linalg.norm(x - x0) <= tol * np.linalg.norm(x0), rep
 
 
+@pytest.mark.parametrize("precision", [64, 32])
+def test_wavefront_and_blocked_lu_solves_agree(precision):
+    """The factor apply's blocked LU solves as one wavefront launch per
+    triangle (solve_wave=1, the default) or one launch pair per 64-row block
+    (solve_wave=0): the same solves in another summation order, so the
+    applies agree to round-off (1e-12 FP64, 1e-5 FP32 relative), eagerly and
+    through repeated graph replays."""
+    b, pde, kw, leaf, prm, ctx, okern, otree = setup("stokes3d_ellipsoid")
+    rng = np.random.default_rng(31)
+    rhs = [rng.normal(size=ctx.ndof) for _ in range(3)]
+    out = {}
+    for wave in (0, 1):
+        for graph in (0, 1):
+            ctx.skeletonize(ki.default_params(precision=precision,
+                                              solve_wave=wave,
+                                              solve_graph=graph, **kw))
+            assert ctx.stats()["top_size"] > 64  # several row blocks
+            out[wave, graph] = [ctx.solve(r) for r in rhs + rhs]
+    tol = 1e-12 if precision == 64 else 1e-5
+    for key, xs in out.items():
+        for x, x0 in zip(xs, out[0, 0]):
+            assert np.linalg.norm(x - x0) <= tol * np.linalg.norm(x0), key
+
+
 def test_gemm_tile_shapes_and_autotuner(monkeypatch):
     """Both tile shapes are correct on a large product, and the autotuner
     records a decision for the class after its four timed launches."""
