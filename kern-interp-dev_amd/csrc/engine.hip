@@ -31,6 +31,7 @@
 #include <memory>
 #include <random>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -166,6 +167,26 @@ struct Ctx {
 };
 
 namespace {
+
+// f(i) for i in [0, n) on up to 16 host threads (static chunks; f must
+// only write state owned by its index).  Small n runs inline.
+template <typename F>
+void parallel_for(int n, F f) {
+  const int hw = std::max(1u, std::thread::hardware_concurrency());
+  const int nt = std::min({hw, 16, n / 8});
+  if (nt <= 1) {
+    for (int i = 0; i < n; i++) f(i);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(nt);
+  for (int t = 0; t < nt; t++)
+    th.emplace_back([&, t] {
+      const int i0 = (int64_t)n * t / nt, i1 = (int64_t)n * (t + 1) / nt;
+      for (int i = i0; i < i1; i++) f(i);
+    });
+  for (auto& x : th) x.join();
+}
 
 double now() {
   return std::chrono::duration<double>(
@@ -356,8 +377,16 @@ void skeletonize_t(Ctx& c) {
     b.fac_level = -1;
   }
   c.drop_solve_graph();  // it refers to the previous factorization's buffers
-  c.lev.clear();
-  for (int l = 0; l < L; l++) c.lev.emplace_back(new LevelDev());
+  // per-level buffers persist across factorizations (growth-only), so a
+  // refactorization allocates nothing new; the plan fields start over
+  c.lev.resize(L);
+  for (int l = 0; l < L; l++) {
+    if (!c.lev[l]) c.lev[l].reset(new LevelDev());
+    LevelDev& LD = *c.lev[l];
+    for (int ph = 0; ph < 4; ph++) LD.item_off[ph] = LD.item_cnt[ph] = 0;
+    LD.nsweep = LD.max_n = LD.max_r = LD.max_k = 0;
+    LD.nfac = 0;
+  }
   c.boxlog.clear();
   c.kclk.clear();
   c.featlog.clear();
@@ -384,6 +413,8 @@ void skeletonize_t(Ctx& c) {
   for (int level = L - 1; level >= 1; level--) {
     // per-level log line (KI_VERBOSE), printed however the level ends
     const double lv_t0 = now();
+    // host-side stamps for the log: ID set-up done, ID done, storage done
+    double lv_t_id = lv_t0, lv_t_sets = lv_t0, lv_t_dec = lv_t0;
     const ki_stats st0 = st;
     int lv_nc = 0, lv_ncomp = 0, lv_maxn = 0, lv_maxm = 0;
     int64_t lv_sumn = 0, lv_summ = 0, lv_sumk = 0;
@@ -395,11 +426,14 @@ void skeletonize_t(Ctx& c) {
       std::fprintf(stderr,
                    "[ki] level %d: id %d comp %d sum_n %lld max_n %d "
                    "sum_m %lld max_m %d sum_k %lld | fill %.4f qr %.4f "
-                   "dec %.4f wall %.4f s\n",
+                   "dec %.4f wall %.4f s | host: setup %.4f sets %.4f "
+                   "storage %.4f\n",
                    level, lv_nc, lv_ncomp, (long long)lv_sumn, lv_maxn,
                    (long long)lv_summ, lv_maxm, (long long)lv_sumk,
                    st.t_fill - st0.t_fill, st.t_qr - st0.t_qr,
-                   st.t_decouple - st0.t_decouple, now() - lv_t0);
+                   st.t_decouple - st0.t_decouple, now() - lv_t0,
+                   lv_t_id - lv_t0, lv_t_sets - lv_t_id,
+                   lv_t_dec - lv_t_sets);
       std::fflush(stderr);
     }};
     tr.remove_inactive_dofs_at_level(level);
@@ -411,21 +445,27 @@ void skeletonize_t(Ctx& c) {
     };
     std::vector<Cand> cands;
     for (int b : tr.levels[level]) {
-      Box& bx = tr.boxes[b];
+      const Box& bx = tr.boxes[b];
       if (bx.compressed ||
           (int)bx.active.size() < c.prm.min_dofs_to_compress)
         continue;
-      Cand cd{b, 0, false, {}};
-      id_rows(c, bx, cd.rows, cd.special);
-      cd.m = 2 * (int)cd.rows.size() + (cd.special ? 0 : P);
-      if (cd.m == 0) continue;
-      cands.push_back(std::move(cd));
+      cands.push_back(Cand{b, 0, false, {}});
     }
+    // the row sets are independent per box: several host threads
+    parallel_for(cands.size(), [&](int q) {
+      Cand& cd = cands[q];
+      id_rows(c, tr.boxes[cd.b], cd.rows, cd.special);
+      cd.m = 2 * (int)cd.rows.size() + (cd.special ? 0 : P);
+    });
+    cands.erase(std::remove_if(cands.begin(), cands.end(),
+                               [](const Cand& cd) { return cd.m == 0; }),
+                cands.end());
     const int ncand = cands.size();
     std::vector<float> feat((size_t)ncand * MLP_NF), pred(ncand, -1.0f);
-    for (int q = 0; q < ncand; q++)
+    parallel_for(ncand, [&](int q) {
       box_features(c, tr.boxes[cands[q].b], cands[q].m, cands[q].special, tol,
                    feat.data() + (size_t)q * MLP_NF);
+    });
     const bool predict = c.prm.use_predictor && c.mlp_ready && ncand > 0;
     if (predict) {
       upload(c.mlp_X, feat, s);
@@ -634,6 +674,7 @@ void skeletonize_t(Ctx& c) {
       };
       std::vector<int> all(nc);
       for (int q = 0; q < nc; q++) all[q] = q;
+      lv_t_id = now();
       run_id(all);
       // A predicted rank cap that the rank rule did not reach: redo the box
       // with the full pivoted factorization.
@@ -666,6 +707,7 @@ void skeletonize_t(Ctx& c) {
                        4.0 * qd[q].n;
       }
     }
+    lv_t_sets = now();
     // ---- ranks -> skeleton / redundant sets ----------------------------
     std::vector<int> comp;  // compressed boxes at this level
     std::vector<int> comp_q;
@@ -802,6 +844,7 @@ void skeletonize_t(Ctx& c) {
     lv_ncomp = comp.size();
     if (comp.empty()) continue;
 
+    lv_t_dec = now();
     // ---- decouple (skel_factorization.cpp:91-127) ----------------------
     Timer tdec(s);
     const int ncp = comp.size();
