@@ -105,7 +105,7 @@ struct Ctx {
   hipStream_t s = nullptr;
   int dom = 0, sol = 0, pde = 0, npts = 0, ndof = 0;
   std::vector<double> pts, nrm, w, curv;
-  DevBuf d_pts, d_nrm, d_w, d_curv, d_diag;
+  DevBuf d_pts, d_nrm, d_w, d_curv, d_diag, d_opsum;
   DevGeom g;
   bool diag_ready = false;
   Tree tree;
@@ -1628,6 +1628,8 @@ int ki_set_geometry(ki_ctx* ctx, int dom, int pde, const double* pts,
     c.g.nrm = c.d_nrm.as<double>();
     c.g.w = c.d_w.as<double>();
     c.g.curv = c.d_curv.as<double>();
+    c.d_opsum.ensure(sizeof(double) * ((size_t)(npts + 255) / 256 + 1));
+    c.g.red = c.d_opsum.as<double>();
     c.g.diag = nullptr;
     c.diag_ready = false;
     c.tree_built = false;

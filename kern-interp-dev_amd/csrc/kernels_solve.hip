@@ -303,11 +303,57 @@ constexpr int MT = 256;
 // 1 / sqrt(x) for x > 0 in FP64: the hardware estimate (v_rsq_f64) refined
 // by two Newton-Raphson steps y <- y (3 - x y^2) / 2, each of which squares
 // the relative error, so the result is at FP64 rounding level.
+// The estimate comes from the FP32 v_rsq_f32 (relative error ~2^-23, much
+// cheaper than the FP64 one) whenever x is in the normal FP32 range; two
+// steps then take it below FP64 rounding (2^-23 -> ~2^-45 -> ~2^-89).
 __device__ __forceinline__ double rsqrt_nr(double x) {
-  double y = __builtin_amdgcn_rsq(x);
+  const float xf = (float)x;
+  double y;
+  if (__builtin_expect(xf >= 1.2e-38f && xf <= 3.0e38f, 1))
+    y = (double)__builtin_amdgcn_rsqf(xf);
+  else
+    y = __builtin_amdgcn_rsq(x);
   y = y * (1.5 - 0.5 * x * y * y);
   y = y * (1.5 - 0.5 * x * y * y);
   return y;
+}
+
+// Stokes double layer, rank-one completion term: its sum over the sources,
+// sum_s w_s (n_s . mu_s) n_t, factors into n_t times one scalar.  Fixed
+// two-stage order (256-point blocks, then the block sums) so every call
+// gives the same bits.
+template <int DOM>
+__global__ void __launch_bounds__(256)
+nmu_partial_kernel(DevGeom g, const double* __restrict__ mu) {
+  __shared__ double sh[256];
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  double v = 0.0;
+  if (j < g.npts) {
+#pragma unroll
+    for (int a = 0; a < DOM; a++)
+      v += g.nrm[(int64_t)j * DOM + a] * mu[(int64_t)j * DOM + a];
+    v *= g.w[j];
+  }
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) g.red[blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(256) nmu_final_kernel(DevGeom g, int nblk) {
+  __shared__ double sh[256];
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += 256) v += g.red[b];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) g.red[nblk] = sh[0];
 }
 
 // Targets are the boundary points themselves (self = 1, the residual
@@ -321,6 +367,9 @@ matvec_kernel(DevGeom g, const double* __restrict__ tpts,
               const double* __restrict__ tnrm, int ntgt, int self,
               const double* __restrict__ mu, double* __restrict__ y) {
   const double PI = 3.14159265358979323846;
+  // kernel constant folded into the staged source weights
+  const double kC = SOL == 1 ? (DOM == 2 ? -1.0 / (2 * PI) : -1.0 / (4 * PI))
+                             : (DOM == 2 ? 1.0 / PI : -3.0 / (4.0 * PI));
   const int dom = DOM, sol = SOL, n = g.npts;
   const int i = blockIdx.x * MT + threadIdx.x;
   __shared__ double sx[MT][DOM], sn[MT][DOM], sw[MT], sm[MT][SOL];
@@ -346,7 +395,7 @@ matvec_kernel(DevGeom g, const double* __restrict__ tpts,
 #pragma unroll
       for (int a = 0; a < SOL; a++)
         sm[threadIdx.x][a] = mu[(int64_t)j * SOL + a];
-      sw[threadIdx.x] = g.w[j];
+      sw[threadIdx.x] = g.w[j] * kC;  // the weight times the kernel constant
     }
     __syncthreads();
     const int cnt = min(MT, n - j0);
@@ -364,29 +413,27 @@ matvec_kernel(DevGeom g, const double* __restrict__ tpts,
       rr = is_self ? 1.0 : rr;
       if (SOL == 1) {
         if (DOM == 2) {
-          acc[0] += -w * (1.0 / (2 * PI)) * dot / rr * sm[q][0];
+          acc[0] += w * dot / rr * sm[q][0];
         } else {
           const double ir = rsqrt_nr(rr);
-          acc[0] += -w * (1.0 / (4 * PI)) * dot * (ir * ir * ir) * sm[q][0];
+          acc[0] += w * dot * (ir * ir * ir) * sm[q][0];
         }
       } else {
-        double rm = 0.0, nm = 0.0;
+        double rm = 0.0;
 #pragma unroll
-        for (int a = 0; a < DOM; a++) {
-          rm += r[a] * sm[q][a];
-          nm += sn[q][a] * sm[q][a];
-        }
+        for (int a = 0; a < DOM; a++) rm += r[a] * sm[q][a];
         double f;
         if (DOM == 2) {
-          f = w * (1 / PI) * dot / (rr * rr);
+          f = w * dot / (rr * rr);
         } else {
           const double ir = rsqrt_nr(rr);
           const double ir2 = ir * ir;
-          f = w * (-3.0 / (4.0 * PI)) * dot * (ir2 * ir2 * ir);
+          f = w * dot * (ir2 * ir2 * ir);
         }
-        const double wnm = w * nm;
+        // the rank-one term w_s (n_s . mu_s) n_t is added after the loop
+        const double frm = f * rm;
 #pragma unroll
-        for (int a = 0; a < SOL; a++) acc[a] += f * r[a] * rm + tn[a] * wnm;
+        for (int a = 0; a < SOL; a++) acc[a] += frm * r[a];
       }
     }
   }
@@ -394,6 +441,16 @@ matvec_kernel(DevGeom g, const double* __restrict__ tpts,
   if (!self) {
     for (int a = 0; a < sol; a++) y[(int64_t)i * sol + a] = acc[a];
     return;
+  }
+  if constexpr (SOL > 1) {
+    // rank-one term: n_t times the sum over the other sources of
+    // w_s (n_s . mu_s) (the whole sum, from nmu_*_kernel, minus this point)
+    double own = 0.0;
+#pragma unroll
+    for (int a = 0; a < DOM; a++) own += g.nrm[(int64_t)i * DOM + a] * mu[(int64_t)i * SOL + a];
+    const double wn = g.red[(n + 255) / 256] - g.w[i] * own;
+#pragma unroll
+    for (int a = 0; a < SOL; a++) acc[a] += tn[a] * wn;
   }
   // self term
   const double w = g.w[i];
@@ -609,6 +666,16 @@ static void matvec_any(const DevGeom& g, const double* tpts,
                        const double* tnrm, int ntgt, int self,
                        const double* mu, double* y, hipStream_t s) {
   const dim3 grid((ntgt + MT - 1) / MT), blk(MT);
+  if (self && g.sol > 1) {  // the rank-one term's source sum first
+    const int nblk = (g.npts + 255) / 256;
+    if (g.dom == 2)
+      hipLaunchKernelGGL(nmu_partial_kernel<2>, dim3(nblk), dim3(256), 0, s, g,
+                         mu);
+    else
+      hipLaunchKernelGGL(nmu_partial_kernel<3>, dim3(nblk), dim3(256), 0, s, g,
+                         mu);
+    hipLaunchKernelGGL(nmu_final_kernel, dim3(1), dim3(256), 0, s, g, nblk);
+  }
   if (g.dom == 2 && g.sol == 1)
     hipLaunchKernelGGL((matvec_kernel<2, 1>), grid, blk, 0, s, g, tpts, tnrm,
                        ntgt, self, mu, y);

@@ -95,6 +95,8 @@ struct DevGeom {
   double* w = nullptr;      // npts
   double* curv = nullptr;   // npts
   double* diag = nullptr;   // 3D: npts (Laplace) or npts*9 (Stokes)
+  double* red = nullptr;    // scratch of the operator's source sums:
+                            // (npts + 255) / 256 + 1 values
 };
 
 // Kernel-entry evaluation mode for one row segment of a generated block.
