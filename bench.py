@@ -301,7 +301,7 @@ def main():
         "gemm_tile_autotuner": [
             {"elem_bytes": t["elem_bytes"], "ta": t["ta"], "tb": t["tb"],
              "m_le": 1 << t["log2_m"], "n_le": 1 << t["log2_n"],
-             "batch_le": 1 << t["log2_batch"],
+             "k_le": 1 << t["log2_k"], "batch_le": 1 << t["log2_batch"],
              "tile": {0: 64, 1: 128}.get(t["tile128"]),
              "t64_over_t128": t["t64_over_t128_x1000"] / 1000.0}
             for t in ki.gemm_tuning()],

@@ -182,10 +182,10 @@ int ki_gemm(ki_ctx* ctx, int precision, int ta, int tb, int m, int n, int k,
             double beta, double* c, int ldc);
 
 /* Decisions of the batched-GEMM tile autotuner in this process: *count
- * classes, 8 ints each {element bytes, ta, tb, ceil log2 max m, ceil log2
- * max n, ceil log2 batch count, tile (0: 64 x 64, 1: 128 x 128, -1: not yet
- * decided), 1000 * time(64) / time(128)}; copies min(cap, count) classes.
- * No reference counterpart. */
+ * classes, 9 ints each {element bytes, ta, tb, ceil log2 max m, ceil log2
+ * max n, ceil log2 max k, ceil log2 batch count, tile (0: 64 x 64, 1:
+ * 128 x 128, -1: not yet decided), 1000 * time(64) / time(128)}; copies
+ * min(cap, count) classes.  No reference counterpart. */
 int ki_gemm_tuning(int* out, int cap, int* count);
 
 /* Factorization statistics. */

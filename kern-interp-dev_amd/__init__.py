@@ -192,10 +192,10 @@ def gemm_tuning() -> list:
     lib = load_library()
     cnt = C.c_int()
     _check(lib.ki_gemm_tuning(None, 0, C.byref(cnt)))
-    out = np.zeros((max(cnt.value, 1), 8), dtype=np.int32)
+    out = np.zeros((max(cnt.value, 1), 9), dtype=np.int32)
     _check(lib.ki_gemm_tuning(_iptr(out), cnt.value, C.byref(cnt)))
-    keys = ("elem_bytes", "ta", "tb", "log2_m", "log2_n", "log2_batch",
-            "tile128", "t64_over_t128_x1000")
+    keys = ("elem_bytes", "ta", "tb", "log2_m", "log2_n", "log2_k",
+            "log2_batch", "tile128", "t64_over_t128_x1000")
     return [dict(zip(keys, map(int, r))) for r in out[:cnt.value]]
 
 

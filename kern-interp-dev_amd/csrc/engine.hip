@@ -955,15 +955,16 @@ void skeletonize_t(Ctx& c) {
     // four GEMMs
     auto gemm_batch = [&](auto make, const T* A, const T* B, T* C) {
       std::vector<GemmDesc> gd(ncp);
-      int mm = 0, nn = 0;
+      int mm = 0, nn = 0, kk = 0;
       for (int q = 0; q < ncp; q++) {
         gd[q] = make(tr.boxes[comp[q]], kb_off[q]);
         mm = std::max(mm, gd[q].m);
         nn = std::max(nn, gd[q].n);
+        kk = std::max(kk, gd[q].k);
         st.flops_schur += 2.0 * gd[q].m * gd[q].n * gd[q].k;
       }
       upload(c.gdesc, gd, s);
-      launch_gemm<T>(c.gdesc.as<GemmDesc>(), ncp, mm, nn, A, B, C, s,
+      launch_gemm<T>(c.gdesc.as<GemmDesc>(), ncp, mm, nn, kk, A, B, C, s,
                      gd[0].ta, gd[0].tb);
       KI_HIP(hipStreamSynchronize(s));
     };
@@ -2058,7 +2059,7 @@ static void gemm_host(ki::Ctx& c, int ta, int tb, int m, int n, int k,
                                     ta, tb, alpha, beta}};
   upload(c.gdesc, gd, c.s);
   T* p = c.tmp.as<T>();
-  launch_gemm<T>(c.gdesc.as<GemmDesc>(), 1, m, n, p, p, p, c.s, ta, tb);
+  launch_gemm<T>(c.gdesc.as<GemmDesc>(), 1, m, n, k, p, p, p, c.s, ta, tb);
   KI_HIP(hipMemcpyAsync(h.data() + na + nb, p + na + nb, sizeof(T) * nc,
                         hipMemcpyDeviceToHost, c.s));
   KI_HIP(hipStreamSynchronize(c.s));
@@ -2094,7 +2095,7 @@ int ki_gemm_tuning(int* out, int cap, int* count) {
     if (count) *count = t.size();
     const int nc = std::max(0, std::min(cap, (int)t.size()));
     for (int i = 0; out && i < nc; i++)
-      for (int j = 0; j < 8; j++) out[8 * i + j] = t[i][j];
+      for (int j = 0; j < 9; j++) out[9 * i + j] = t[i][j];
   });
 }
 

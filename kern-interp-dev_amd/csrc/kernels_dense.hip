@@ -982,12 +982,16 @@ scatter_kernel(const ScatterDesc* __restrict__ desc, const TS* __restrict__ src_
 }  // namespace
 
 // Tile-shape autotuner for the batched GEMM.  A product class is (element
-// size, ta, tb, ceil log2 of max m, max n and the batch count).  Its first
+// size, ta, tb, ceil log2 of max m, max n, max k and the batch count).  Its
+// first
 // four launches alternate the 64 x 64 and the 128 x 128 kernel on the real
 // data (both compute the same product) under HIP events, each shape's time
 // being the faster of its two runs (the first run of a kernel can carry
 // one-time costs); every later launch of the class uses the faster shape.
 // Classes narrower than 256 rows or columns use the 64 x 64 kernel untimed.
+// Products with K >= 256 (and both sides >= 256) always take the 128 tile
+// and those with K < 64 the 64 tile (2 k-slices leave the 128 tile mostly
+// prologue and epilogue); only the band between is timed online.
 // KI_GEMM_TILE = 64 / 128 forces a shape.
 struct TileChoice {
   float ms[2] = {-1.f, -1.f};
@@ -1005,14 +1009,15 @@ static int ceil_log2(int x) {
 static std::mutex g_tune_mu;
 static std::map<uint64_t, TileChoice> g_tune;
 
-std::vector<std::array<int, 8>> gemm_tuning_table() {
+std::vector<std::array<int, 9>> gemm_tuning_table() {
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  std::vector<std::array<int, 8>> out;
+  std::vector<std::array<int, 9>> out;
   for (const auto& kv : g_tune) {
     const uint64_t k = kv.first;
     out.push_back({(int)(k >> 40 & 0xff), (int)(k >> 32 & 0xff),
                    (int)(k >> 24 & 0xff), (int)(k >> 16 & 0xff),
-                   (int)(k >> 8 & 0xff), (int)(k & 0xff), kv.second.best,
+                   (int)(k >> 8 & 0xff), (int)(k >> 48 & 0xff),
+                   (int)(k & 0xff), kv.second.best,
                    (int)(1000 * (kv.second.ms[1] > 0 && kv.second.ms[0] > 0
                                      ? kv.second.ms[0] / kv.second.ms[1]
                                      : 0))});
@@ -1022,7 +1027,8 @@ std::vector<std::array<int, 8>> gemm_tuning_table() {
 
 template <typename T>
 void launch_gemm(const GemmDesc* d_desc, int ndesc, int max_m, int max_n,
-                 const T* a, const T* b, T* c, hipStream_t s, int ta, int tb) {
+                 int max_k, const T* a, const T* b, T* c, hipStream_t s,
+                 int ta, int tb) {
   if (ndesc == 0 || max_m == 0 || max_n == 0) return;
   bool big = false;
   int tuning = -1;  // variant being timed by this launch
@@ -1030,8 +1036,13 @@ void launch_gemm(const GemmDesc* d_desc, int ndesc, int max_m, int max_n,
   const char* force = std::getenv("KI_GEMM_TILE");
   if (force) {
     big = std::atoi(force) == 128;
-  } else if (max_m >= 256 && max_n >= 256) {
-    const uint64_t key = (uint64_t)sizeof(T) << 40 | (uint64_t)ta << 32 |
+  } else if (max_m >= 256 && max_n >= 256 && max_k >= 256) {
+    big = true;  // deep products: the 128 tile measured 2x faster or more
+  } else if (max_m >= 256 && max_n >= 256 && max_k >= 64) {
+    // the band where either tile can win is tuned online; K is part of the
+    // class (a K = 64 update and a K = 200 product differ)
+    const uint64_t key = (uint64_t)ceil_log2(std::max(max_k, 1)) << 48 |
+                         (uint64_t)sizeof(T) << 40 | (uint64_t)ta << 32 |
                          (uint64_t)tb << 24 |
                          (uint64_t)ceil_log2(max_m) << 16 |
                          (uint64_t)ceil_log2(max_n) << 8 |
@@ -1274,7 +1285,7 @@ static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
         if (!library_gemm<T>(q, a, a, a, s)) {
           KI_HIP(hipMemcpyAsync(d_g, &q, sizeof(GemmDesc),
                                 hipMemcpyHostToDevice, s));
-          launch_gemm<T>(d_g, 1, q.m, q.n, a, a, a, s, 0, 0);
+          launch_gemm<T>(d_g, 1, q.m, q.n, q.k, a, a, a, s, 0, 0);
           KI_HIP(hipStreamSynchronize(s));  // q dies with this scope
         }
       }
@@ -1352,7 +1363,7 @@ static void lu_two_level(const MatDesc& x, T* a, int* piv_all, int start,
         hipLaunchKernelGGL(trsm_small_cols<T>, cols(c1 - c0), dim3(256), 0, s,
                            A, ld, j0, jb, c0, c1);
         const GemmDesc& q = gd[gi];
-        launch_gemm<T>(d_g + gi, 1, q.m, q.n, a, a, a, s, 0, 0);
+        launch_gemm<T>(d_g + gi, 1, q.m, q.n, q.k, a, a, a, s, 0, 0);
         gi++;
       }
     }
@@ -1380,7 +1391,7 @@ static void lu_two_level(const MatDesc& x, T* a, int* piv_all, int start,
         // tuned kernel when it is available, else the batched one here
         const GemmDesc& q = gd[gi];
         if (!library_gemm<T>(q, a, a, a, s))
-          launch_gemm<T>(d_g + gi, 1, q.m, q.n, a, a, a, s, 0, 0);
+          launch_gemm<T>(d_g + gi, 1, q.m, q.n, q.k, a, a, a, s, 0, 0);
         gi++;
       }
     }
@@ -1457,8 +1468,8 @@ void batched_lu(const std::vector<MatDesc>& h, MatDesc* d_desc, T* a, int* piv,
     hipLaunchKernelGGL(lu_swap_trsm<T>, dim3((maxn + 7) / 8, nb),
                        dim3(256), 0, s, d_desc, a, piv, j0);
     if (mm[p] > 0 && nn[p] > 0)
-      launch_gemm<T>(d_g + (size_t)p * nb, nb, mm[p], nn[p], a, a, a, s, 0,
-                     0);
+      launch_gemm<T>(d_g + (size_t)p * nb, nb, mm[p], nn[p], LB, a, a, a, s,
+                     0, 0);
   }
   KI_HIP(hipGetLastError());
   if (cp_live) cp.finish(s);
@@ -1566,8 +1577,8 @@ static void trsm_sweeps(const std::vector<MatDesc>& h, MatDesc* d_desc,
                              a, b, blk);
       }
       if (mm[slot] > 0 && nn[slot] > 0)
-        launch_gemm<T>(d_g + (size_t)slot * nb, nb, mm[slot], nn[slot], a, b,
-                       b, s, modes[mi] >= 2 ? 1 : 0, 0);
+        launch_gemm<T>(d_g + (size_t)slot * nb, nb, mm[slot], nn[slot], BW, a,
+                       b, b, s, modes[mi] >= 2 ? 1 : 0, 0);
     }
   }
   if (swap == 2)
@@ -1643,8 +1654,9 @@ void launch_scatter(const ScatterDesc* d_desc, int ndesc, int max_n,
 }
 
 #define KI_INST(T)                                                            \
-  template void launch_gemm<T>(const GemmDesc*, int, int, int, const T*,      \
-                               const T*, T*, hipStream_t, int, int);          \
+  template void launch_gemm<T>(const GemmDesc*, int, int, int, int,          \
+                               const T*, const T*, T*, hipStream_t, int,      \
+                               int);                                          \
   template void batched_lu<T>(const std::vector<MatDesc>&, MatDesc*, T*,      \
                               int*, hipStream_t);                             \
   template void batched_trsm_upper<T>(const std::vector<MatDesc>&, MatDesc*,  \

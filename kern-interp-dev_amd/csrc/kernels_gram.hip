@@ -783,7 +783,7 @@ void pchol_large(const std::vector<CholDesc>& h, const CholDesc* d_desc,
       }
     }
     if (!lib)
-      launch_gemm<double>(d_g + base, nb, maxn, maxn, ws, ws, g, s, 0, 1);
+      launch_gemm<double>(d_g + base, nb, maxn, maxn, CB, ws, ws, g, s, 0, 1);
   }
   if (xch != nullptr) {
     int hf = 0;

@@ -231,13 +231,15 @@ bool library_gemm(const GemmDesc& q, const T* a, const T* b, T* c,
 // The GEMM tile autotuner's decisions so far: per product class {element
 // bytes, ta, tb, log2 max m, log2 max n, log2 batch, chosen tile (0: 64,
 // 1: 128, -1: undecided), 1000 * t64 / t128}.
-std::vector<std::array<int, 8>> gemm_tuning_table();
+std::vector<std::array<int, 9>> gemm_tuning_table();
 
-// ta / tb: the (common) transposition flags of every descriptor in the
-// launch; together with the sizes they key the tile-shape autotuner.
+// max_m / max_n / max_k: the largest descriptor sizes; ta / tb: the
+// (common) transposition flags of every descriptor in the launch.  Sizes,
+// flags and batch count key the tile-shape autotuner.
 template <typename T>
 void launch_gemm(const GemmDesc* d_desc, int ndesc, int max_m, int max_n,
-                 const T* a, const T* b, T* c, hipStream_t s, int ta, int tb);
+                 int max_k, const T* a, const T* b, T* c, hipStream_t s,
+                 int ta, int tb);
 
 struct MatDesc {       // one square (or m x n) matrix for LU / TRSM / copy
   int m, n;

@@ -436,22 +436,33 @@ def test_wavefront_and_blocked_lu_solves_agree(precision):
 
 
 def test_gemm_tile_shapes_and_autotuner(monkeypatch):
-    """Both tile shapes are correct on a large product, and the autotuner
-    records a decision for the class after its four timed launches."""
+    """Both tile shapes are correct on a large product; products of depth
+    64..255 are timed online per class (output size, depth, batch) and get
+    a decision after four launches, while deeper and shallower ones take
+    their fixed shape untimed."""
     rng = np.random.default_rng(5)
-    a = rng.standard_normal((900, 300))
-    b = rng.standard_normal((300, 700))
-    want = a @ b
     ctx = ki.Context()
+
+    def check(m, n, k):
+        a = rng.standard_normal((m, k))
+        b = rng.standard_normal((k, n))
+        got = ctx.gemm(a, b)
+        want = a @ b
+        assert np.linalg.norm(got - want) <= 1e-13 * np.linalg.norm(want)
+
     for tile in ("64", "128"):
         monkeypatch.setenv("KI_GEMM_TILE", tile)
-        got = ctx.gemm(a, b)
-        assert np.linalg.norm(got - want) <= 1e-13 * np.linalg.norm(want)
+        check(900, 700, 300)
     monkeypatch.delenv("KI_GEMM_TILE")
     for _ in range(5):
-        got = ctx.gemm(a, b)
-        assert np.linalg.norm(got - want) <= 1e-13 * np.linalg.norm(want)
-    table = ki.gemm_tuning()
-    mine = [t for t in table if t["elem_bytes"] == 8 and t["log2_m"] == 10
-            and t["log2_n"] == 10 and t["log2_batch"] == 0]
-    assert mine and mine[0]["tile128"] in (0, 1)
+        check(900, 700, 100)
+    check(900, 700, 200)
+    check(900, 700, 300)
+    check(900, 700, 40)
+    table = [t for t in ki.gemm_tuning() if t["elem_bytes"] == 8
+             and t["log2_m"] == 10 and t["log2_n"] == 10
+             and t["log2_batch"] == 0]
+    depth = {t["log2_k"]: t for t in table}
+    assert depth[7]["tile128"] in (0, 1)   # K = 100: decided
+    assert 8 in depth                      # K = 200: its own class
+    assert 9 not in depth and 6 not in depth  # K = 300, K = 40: fixed
