@@ -188,15 +188,29 @@ struct SliceReader {
   }
 };
 
-template <typename T, int TA, int TB>
+// SYM: C is symmetric (m = n) and so is the product (B = A, TB = !TA): only
+// the upper tiles run, each writing its block and the mirrored one.
+template <typename T, int TA, int TB, bool SYM = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 gemm128_kernel(const GemmDesc* __restrict__ desc, const T* __restrict__ a_all,
                const T* __restrict__ b_all, T* __restrict__ c_all) {
   const GemmDesc d = desc[blockIdx.y];
   const int tiles_m = (d.m + H - 1) / H;
   const int tiles_n = (d.n + H - 1) / H;
-  if ((int)blockIdx.x >= tiles_m * tiles_n) return;
-  const int i0 = (blockIdx.x % tiles_m) * H, j0 = (blockIdx.x / tiles_m) * H;
+  int i0, j0;
+  if (SYM) {
+    const int t = blockIdx.x;
+    if (t >= tiles_n * (tiles_n + 1) / 2) return;
+    int tj = (int)((sqrt(8.0 * t + 1.0) - 1.0) / 2.0);
+    while ((tj + 1) * (tj + 2) / 2 <= t) tj++;
+    while (tj * (tj + 1) / 2 > t) tj--;
+    i0 = (t - tj * (tj + 1) / 2) * H;
+    j0 = tj * H;
+  } else {
+    if ((int)blockIdx.x >= tiles_m * tiles_n) return;
+    i0 = (blockIdx.x % tiles_m) * H;
+    j0 = (blockIdx.x / tiles_m) * H;
+  }
   const T* A = a_all + d.a_off;
   const T* B = b_all + d.b_off;
   T* C = c_all + d.c_off;
@@ -251,7 +265,9 @@ gemm128_kernel(const GemmDesc* __restrict__ desc, const T* __restrict__ a_all,
         if (gi >= d.m || gj >= d.n) continue;
         T* cp = C + gi + (int64_t)d.ldc * gj;
         const T x = acc[a][b][v];
-        *cp = beta == (T)0 ? alpha * x : alpha * x + beta * *cp;
+        const T nv = beta == (T)0 ? alpha * x : alpha * x + beta * *cp;
+        *cp = nv;
+        if (SYM && i0 != j0) C[gj + (int64_t)d.ldc * gi] = nv;
       }
     }
   }
@@ -1652,6 +1668,22 @@ void launch_scatter(const ScatterDesc* d_desc, int ndesc, int max_n,
   }
   KI_HIP(hipGetLastError());
 }
+
+template <typename T>
+void launch_sym_update(const GemmDesc* d_desc, int ndesc, int max_n,
+                       const T* a, T* c, hipStream_t s) {
+  if (ndesc == 0 || max_n == 0) return;
+  const int tn = (max_n + H - 1) / H;
+  for (int base = 0; base < ndesc; base += 65535) {
+    const int cnt = std::min(65535, ndesc - base);
+    hipLaunchKernelGGL((gemm128_kernel<T, 0, 1, true>),
+                       dim3(tn * (tn + 1) / 2, cnt), dim3(256), 0, s,
+                       d_desc + base, a, a, c);
+  }
+  KI_HIP(hipGetLastError());
+}
+template void launch_sym_update<double>(const GemmDesc*, int, int,
+                                        const double*, double*, hipStream_t);
 
 #define KI_INST(T)                                                            \
   template void launch_gemm<T>(const GemmDesc*, int, int, int, int,          \

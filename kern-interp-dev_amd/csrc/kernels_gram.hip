@@ -768,22 +768,10 @@ void pchol_large(const std::vector<CholDesc>& h, const CholDesc* d_desc,
       stop = all_done();
     }
     if (stop || b1 >= maxsteps) break;
-    // S -= L_blk L_blk^T is a plain GEMM per box: for a few large boxes the
-    // library's tuned kernel, else one batched launch
-    const size_t base = (size_t)bk * nb;
-    bool lib = nb <= 128 && maxn >= 512, any = false;
-    for (int b = 0; lib && b < nb; b++) {
-      const GemmDesc& q = gd[base + b];
-      if (q.m == 0) continue;
-      if (library_gemm<double>(q, ws, ws, g, s)) {
-        any = true;
-      } else {
-        KI_CHECK(!any, 3, "rocBLAS GEMM failed after earlier successes");
-        lib = false;
-      }
-    }
-    if (!lib)
-      launch_gemm<double>(d_g + base, nb, maxn, maxn, CB, ws, ws, g, s, 0, 1);
+    // S -= L_blk L_blk^T for every box in one launch: S stays exactly
+    // symmetric, so only the upper tiles are computed and each is mirrored
+    // (half the flops of the full product)
+    launch_sym_update<double>(d_g + (size_t)bk * nb, nb, maxn, ws, g, s);
   }
   if (xch != nullptr) {
     int hf = 0;

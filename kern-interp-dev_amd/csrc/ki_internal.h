@@ -236,6 +236,11 @@ std::vector<std::array<int, 9>> gemm_tuning_table();
 // max_m / max_n / max_k: the largest descriptor sizes; ta / tb: the
 // (common) transposition flags of every descriptor in the launch.  Sizes,
 // flags and batch count key the tile-shape autotuner.
+// C = alpha A A^T + beta C for symmetric C (descriptors with m = n, ta = 0,
+// tb = 1, b = a): the upper 128 x 128 tiles only, each mirrored.
+template <typename T>
+void launch_sym_update(const GemmDesc* d_desc, int ndesc, int max_n,
+                       const T* a, T* c, hipStream_t s);
 template <typename T>
 void launch_gemm(const GemmDesc* d_desc, int ndesc, int max_m, int max_n,
                  int max_k, const T* a, const T* b, T* c, hipStream_t s,
