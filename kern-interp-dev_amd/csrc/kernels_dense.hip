@@ -1271,40 +1271,66 @@ static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
     GemmDesc* d_g = nullptr;
     KI_HIP(hipMallocAsync((void**)&d_g, sizeof(GemmDesc), s));
     auto cols = [](int c) { return dim3((std::max(c, 1) + 255) / 256); };
-    for (; J0 < mn; J0 += BW) {
-      const int JB = std::min(BW, mn - J0);
-      if (!bp.panel(a, piv_all, J0, J0 + JB, s)) break;
-      if (J0 > 0)
-        hipLaunchKernelGGL(swap_cols_range<T>, cols(J0), dim3(256), 0, s, A,
-                           ld, (const int*)piv, J0, J0 + JB, 0, J0);
-      if (n <= J0 + JB) continue;
-      hipLaunchKernelGGL(swap_cols_range<T>, cols(n - J0 - JB), dim3(256), 0,
-                         s, A, ld, (const int*)piv, J0, J0 + JB, J0 + JB, n);
-      // U12 = L11^{-1} A12 (unit lower JB x JB)
-      hipLaunchKernelGGL(trsm_block_row<T>, dim3((n - J0 - JB + 7) / 8),
-                         dim3(256), 0, s, A, ld, J0, JB, J0 + JB, n);
-      if (m > J0 + JB) {
-        // A22 -= L21 U12: a plain K = JB GEMM, the library's when available
-        const GemmDesc q{m - J0 - JB,
-                         n - J0 - JB,
-                         JB,
-                         x.off + (J0 + JB) + (int64_t)ld * J0,
-                         x.off + J0 + (int64_t)ld * (J0 + JB),
-                         x.off + (J0 + JB) + (int64_t)ld * (J0 + JB),
-                         ld,
-                         ld,
-                         ld,
-                         0,
-                         0,
-                         -1.0,
-                         1.0};
-        if (!library_gemm<T>(q, a, a, a, s)) {
-          KI_HIP(hipMemcpyAsync(d_g, &q, sizeof(GemmDesc),
-                                hipMemcpyHostToDevice, s));
-          launch_gemm<T>(d_g, 1, q.m, q.n, q.k, a, a, a, s, 0, 0);
-          KI_HIP(hipStreamSynchronize(s));  // q dies with this scope
-        }
+    auto swaps = [&](int r0, int r1, int c0, int c1) {
+      if (c1 > c0)
+        hipLaunchKernelGGL(swap_cols_range<T>, cols(c1 - c0), dim3(256), 0,
+                           s, A, ld, (const int*)piv, r0, r1, c0, c1);
+    };
+    auto trsm = [&](int j0, int jb, int c0, int c1) {
+      if (c1 > c0)
+        hipLaunchKernelGGL(trsm_block_row<T>, dim3((c1 - c0 + 7) / 8),
+                           dim3(256), 0, s, A, ld, j0, jb, c0, c1);
+    };
+    // A[r0:r0+rows, c0:c0+ncol] -= A[r0:, k0:k0+kk] A[k0:k0+kk, c0:]: a
+    // plain GEMM, the library's when available
+    auto update = [&](int r0, int rows, int c0, int ncol, int k0, int kk) {
+      if (rows <= 0 || ncol <= 0 || kk <= 0) return;
+      const GemmDesc q{rows, ncol, kk,
+                       x.off + r0 + (int64_t)ld * k0,
+                       x.off + k0 + (int64_t)ld * c0,
+                       x.off + r0 + (int64_t)ld * c0,
+                       ld, ld, ld, 0, 0, -1.0, 1.0};
+      if (!library_gemm<T>(q, a, a, a, s)) {
+        KI_HIP(hipMemcpyAsync(d_g, &q, sizeof(GemmDesc),
+                              hipMemcpyHostToDevice, s));
+        launch_gemm<T>(d_g, 1, q.m, q.n, q.k, a, a, a, s, 0, 0);
+        KI_HIP(hipStreamSynchronize(s));  // q dies with this scope
       }
+    };
+    // outer blocks of two panels: each panel is factored whole by one
+    // cooperative launch; the trailing update waits for the outer block,
+    // so it is one K = 2 BW product
+    for (; J0 < mn; J0 += 2 * BW) {
+      const int OE = std::min(J0 + 2 * BW, mn);
+      const int PA = std::min(BW, OE - J0);  // first panel [J0, J0 + PA)
+      const int B0 = J0 + PA, PB = OE - B0;  // second panel [B0, OE)
+      if (!bp.panel(a, piv_all, J0, B0, s)) break;
+      swaps(J0, B0, 0, J0);
+      swaps(J0, B0, B0, n);
+      if (PB > 0) {
+        // the second panel's columns take the first panel's update
+        trsm(J0, PA, B0, OE);
+        update(B0, m - B0, B0, PB, J0, PA);
+        if (!bp.panel(a, piv_all, B0, OE, s)) {
+          // finish the first panel's update of the columns right of the
+          // outer block, then continue in the two-level form from B0
+          trsm(J0, PA, OE, n);
+          update(B0, m - B0, OE, n - OE, J0, PA);
+          J0 = B0;
+          break;
+        }
+        swaps(B0, OE, 0, B0);
+        swaps(B0, OE, OE, n);
+      }
+      if (n <= OE) continue;
+      // U12 of the outer block: rows of the first panel, then those of the
+      // second (after the first's contribution), over the columns right of it
+      trsm(J0, PA, OE, n);
+      if (PB > 0) {
+        update(B0, PB, OE, n - OE, J0, PA);
+        trsm(B0, PB, OE, n);
+      }
+      update(OE, m - OE, OE, n - OE, J0, OE - J0);
     }
     KI_HIP(hipGetLastError());
     KI_HIP(hipStreamSynchronize(s));
