@@ -37,11 +37,16 @@ PEAK_FP32 = 157.3e12       # FLOP/s dense FP32 (vector = matrix on MI355X)
 PEAK_FP64 = 78.6e12
 
 
+# ki_params::id_sketch of the FP32 form (0: the library's default)
+ID_SKETCH = 0
+
+
 def params_for(ki, pb, precision, predictor=0):
     return ki.default_params(id_tol=pb.id_tol, proxy_points=pb.proxy_points,
                              proxy_ratio=pb.proxy_ratio,
                              proxy_kind=1 if pb.boundary.dom == 3 else 0,
-                             precision=precision, use_predictor=predictor)
+                             precision=precision, use_predictor=predictor,
+                             id_sketch=ID_SKETCH if precision == 32 else 0)
 
 
 # Which peak bounds each event-timed kernel: the Gram product runs on the
@@ -181,12 +186,17 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--id-sketch", type=int, default=0,
+                    help="row sketch of the FP32 form's Gram ID: rows per "
+                         "column (0: library default, -1: off)")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-predictor", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 1 GiB read + 1 GiB write elementwise "
                     "kernel (scale for the HBM byte counters) and exit")
     args = ap.parse_args()
+    global ID_SKETCH
+    ID_SKETCH = args.id_sketch
     if args.calibrate_copy:
         calibrate_copy()
         return

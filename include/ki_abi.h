@@ -73,7 +73,13 @@ typedef struct ki_params {
   int solve_wave;           /* 1: the factor apply's blocked LU solves run
                                as one wavefront launch per triangle; 0: one
                                launch pair per 64-row block                  */
-  int reserved[2];
+  int id_sketch;            /* Gram-form ID on a row sketch Y = S A with
+                               id_sketch * n rows (S: random balanced row
+                               partition with random signs) for boxes with
+                               at least twice that many ID rows.  0: auto
+                               (3 in FP32 mode, off in FP64 mode); -1: off;
+                               k >= 1: k * n rows in either mode            */
+  int reserved[1];
 } ki_params;
 
 void ki_params_default(ki_params* p);

@@ -58,7 +58,8 @@ class KiParams(C.Structure):
         ("chol_coop", C.c_int),
         ("lu_block", C.c_int),
         ("solve_wave", C.c_int),
-        ("reserved", C.c_int * 2),
+        ("id_sketch", C.c_int),
+        ("reserved", C.c_int * 1),
     ]
 
 

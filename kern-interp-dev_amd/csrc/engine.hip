@@ -128,6 +128,8 @@ struct Ctx {
   std::vector<int> allskel;
   DevBuf top, top_piv, d_allskel, top_vec, top_desc;
   // scratch
+  DevBuf proxy_tab;  // unit proxy surface (device_proxy)
+  DevBuf rowtab_d, rowtab_i;  // sketch row table of an ID launch (RowTab)
   DevBuf idm, kbn, qws, qpiv, qrank, idxb, fdesc, qdesc, gdesc, mdesc, sdesc,
       mapb, tmp, gram, cws, cstate, cdesc;
   // vectors
@@ -243,6 +245,48 @@ ProxySpec make_proxy(const ki_params& p, int dom) {
       px.gl_w[i] = M_PI / 2 * wx;
     }
   }
+  return px;
+}
+
+// The proxy surface of radius 1 about the origin, 4 doubles per point (unit
+// point = outward normal, weight), uploaded for the fill kernels:
+//   2D     q-th of npts equispaced angles 2 pi q / npts, weight 2 pi / npts;
+//   3D GL  theta_i = 2 pi i / ntheta, phi_j = GL node j on [0, pi],
+//          weight gl_w_j sin(phi_j) 2 pi / ntheta (kernel.h:49-65);
+//   3D Fibonacci  z = 1 - (2q + 1) / npts, azimuth pi (3 - sqrt 5) q,
+//          weight 4 pi / npts.
+ProxySpec device_proxy(Ctx& c, const ki_params& p) {
+  ProxySpec px = make_proxy(p, c.dom);
+  std::vector<double> t(4 * (size_t)px.npts);
+  for (int q = 0; q < px.npts; q++) {
+    double* u = t.data() + 4 * (size_t)q;
+    if (c.dom == 2) {
+      const double ang = 2 * M_PI * q * (1.0 / px.npts);
+      u[0] = std::cos(ang);
+      u[1] = std::sin(ang);
+      u[2] = 0.0;
+      u[3] = 2.0 * M_PI / px.npts;
+    } else if (px.kind == 0) {
+      const int i = q / px.nphi, j = q % px.nphi;
+      const double th = 2 * M_PI * i * (1.0 / px.ntheta);
+      const double ph = px.gl_phi[j];
+      const double sp = std::sin(ph);
+      u[0] = sp * std::cos(th);
+      u[1] = sp * std::sin(th);
+      u[2] = std::cos(ph);
+      u[3] = (px.gl_w[j] * sp) * (2 * M_PI / px.ntheta);
+    } else {
+      const double z = 1.0 - (2.0 * q + 1.0) / px.npts;
+      const double rho = std::sqrt(std::fmax(0.0, 1.0 - z * z));
+      const double th = M_PI * (3.0 - std::sqrt(5.0)) * q;
+      u[0] = rho * std::cos(th);
+      u[1] = rho * std::sin(th);
+      u[2] = z;
+      u[3] = 4.0 * M_PI / px.npts;
+    }
+  }
+  upload(c.proxy_tab, t, c.s);
+  px.tab = c.proxy_tab.as<double>();
   return px;
 }
 
@@ -395,7 +439,7 @@ void skeletonize_t(Ctx& c) {
   ki_stats& st = c.stats;
   st = ki_stats{};
   st.levels = L;
-  const ProxySpec px = make_proxy(c.prm, d);
+  const ProxySpec px = device_proxy(c, c.prm);
   const int P = 2 * sol * px.npts;
   const double t_start = now();
   std::vector<int> pos(c.ndof, -1);  // DOF -> position scratch
@@ -408,6 +452,12 @@ void skeletonize_t(Ctx& c) {
   // The Gram form resolves ratios down to ~1.5e-8 in FP64.
   const bool use_gram =
       c.prm.id_method == 2 || (c.prm.id_method == 0 && tol >= 1e-7);
+  // Row sketch for the Gram form (ki_params::id_sketch): 0 = 3 rows per
+  // column in FP32 mode, off in FP64 mode
+  const int sketch_mult =
+      !use_gram ? 0
+      : c.prm.id_sketch > 0 ? c.prm.id_sketch
+      : (c.prm.id_sketch == 0 && sizeof(T) == 4) ? 3 : 0;
   std::vector<int64_t> gofs;
 
   for (int level = L - 1; level >= 1; level--) {
@@ -442,14 +492,16 @@ void skeletonize_t(Ctx& c) {
       int b, m;
       bool special;
       std::vector<int> rows;
+      int pos;  // index of the box within its level
     };
     std::vector<Cand> cands;
-    for (int b : tr.levels[level]) {
+    for (size_t pl = 0; pl < tr.levels[level].size(); pl++) {
+      const int b = tr.levels[level][pl];
       const Box& bx = tr.boxes[b];
       if (bx.compressed ||
           (int)bx.active.size() < c.prm.min_dofs_to_compress)
         continue;
-      cands.push_back(Cand{b, 0, false, {}});
+      cands.push_back(Cand{b, 0, false, {}, (int)pl});
     }
     // the row sets are independent per box: several host threads
     parallel_for(cands.size(), [&](int q) {
@@ -481,7 +533,9 @@ void skeletonize_t(Ctx& c) {
     std::vector<FillDesc> fd;
     std::vector<QrDesc> qd;
     std::vector<int> cand_feat;  // candidate slot of each ID'd box
+    std::vector<int> grows;      // rows of the matrix the Gram reads
     int64_t idm_el = 0, ws_el = 0, piv_el = 0;
+    int64_t tab_rows = 0;        // sketch row table size (sketched boxes)
     int max_cols = 0, max_n = 0;
     for (int qc = 0; qc < ncand; qc++) {
       const Cand& cd = cands[qc];
@@ -529,9 +583,20 @@ void skeletonize_t(Ctx& c) {
         for (int q = 0; q < 3; q++) f.center[q] = bx.center[q];
         f.radius = (bx.side / 2) * std::sqrt((double)d) * c.prm.proxy_ratio;
       }
+      // Gram form on a row sketch with sketch_mult * n rows when the box has
+      // at least twice that many ID rows
+      const int bk = sketch_mult * n;
+      f.buckets = (bk > 0 && m >= 2 * bk) ? bk : 0;
+      f.key = ((uint32_t)level << 20) | (uint32_t)cd.pos;
+      if (f.buckets > 0) {
+        f.tab_off = tab_rows;
+        tab_rows += m;
+      }
+      const int gr = f.buckets > 0 ? f.buckets : m;
       f.out_off = idm_el;
-      f.ld = m;
+      f.ld = gr;
       fd.push_back(f);
+      grows.push_back(gr);
       QrDesc q{};
       q.m = m;
       q.n = n;
@@ -541,7 +606,7 @@ void skeletonize_t(Ctx& c) {
       q.max_rank = kcap;
       qd.push_back(q);
       bx.id_rows = m;
-      idm_el += (int64_t)m * n;
+      idm_el += (int64_t)gr * n;
       piv_el += n;
       ws_el += 2 * n;
       max_cols = std::max(max_cols, n);
@@ -584,21 +649,31 @@ void skeletonize_t(Ctx& c) {
       auto run_id = [&](const std::vector<int>& sel) {
         std::vector<FillDesc> sf;
         std::vector<QrDesc> sq;
-        int mc = 0, mr = 0;
+        int mc = 0, mr = 0, msk = 0;
         double el = 0.;
         for (int q : sel) {
           sf.push_back(fd[q]);
           sq.push_back(qd[q]);
           mc = std::max(mc, qd[q].n);
-          mr = std::max(mr, qd[q].m);
-          el += (double)qd[q].m * qd[q].n;
+          mr = std::max(mr, grows[q]);
+          if (fd[q].buckets > 0) msk = std::max(msk, qd[q].m);
+          el += (double)grows[q] * qd[q].n;
         }
         const int ns = sel.size();
         upload(c.fdesc, sf, s);
         {
           Timer tm(s);
+          RowTab rt{};
+          if (msk > 0) {
+            c.rowtab_d.ensure(sizeof(double) * 8 * tab_rows);
+            c.rowtab_i.ensure(sizeof(int) * 2 * tab_rows);
+            rt = RowTab{c.rowtab_d.as<double>(), c.rowtab_i.as<int>(),
+                        tab_rows};
+            launch_sketch_rows(c.g, px, c.fdesc.as<FillDesc>(), ns, msk,
+                               c.idxb.as<int>(), rt, s);
+          }
           launch_fill<T>(c.g, px, c.fdesc.as<FillDesc>(), ns, mc, mr,
-                         c.idxb.as<int>(), c.idm.as<T>(), s);
+                         c.idxb.as<int>(), c.idm.as<T>(), s, rt);
           const double dt = tm.stop();
           st.t_fill += dt;
           c.clock("fill_kernel (ID matrices)", dt, 0., sizeof(T) * el);
@@ -627,7 +702,7 @@ void skeletonize_t(Ctx& c) {
         std::vector<CholDesc> small, large;
         for (int q : sel) {
           const int n = qd[q].n;
-          gd.push_back(GramDesc{qd[q].m, n, qd[q].a_off, qd[q].m, gofs[q]});
+          gd.push_back(GramDesc{grows[q], n, qd[q].a_off, grows[q], gofs[q]});
           CholDesc ch{n,       qd[q].max_rank, q, gofs[q], wof[q],
                       wof[q] + (int64_t)n * n, qd[q].piv_off};
           // one workgroup per box only for small boxes; above that the
@@ -700,7 +775,7 @@ void skeletonize_t(Ctx& c) {
         c.predlog.push_back(pred[qc]);
       }
       for (int q = 0; q < nc; q++) {
-        const int64_t mn = (int64_t)qd[q].m * qd[q].n;
+        const int64_t mn = (int64_t)grows[q] * qd[q].n;
         st.bytes_fill += sizeof(T) * (double)mn;
         const int k = rank[q];
         st.bytes_qr += sizeof(T) * ((double)mn + (double)k * (qd[q].n - k)) +
@@ -1759,6 +1834,8 @@ int ki_skeletonize(ki_ctx* ctx, const ki_params* params) {
     KI_CHECK(c.prm.proxy_points > 0, KI_E_ARG, "proxy_points must be > 0");
     KI_CHECK(c.prm.lu_wide_rows >= 0 && c.prm.lu_coop_rows >= 0, KI_E_ARG,
              "LU row thresholds must be >= 0");
+    KI_CHECK(c.prm.id_sketch >= -1 && c.prm.id_sketch <= 64, KI_E_ARG,
+             "id_sketch must be -1, 0 or a row multiple 1..64");
     c.factored = false;
     c.prec = c.prm.precision;
     struct Thresholds {  // this call's LU thresholds, defaults afterwards
@@ -1923,7 +2000,8 @@ int ki_proxy_block(ki_ctx* ctx, const ki_params* params, const double* center,
     Ctx& c = ctx->c;
     KI_CHECK(c.npts > 0, KI_E_STATE, "no geometry");
     KI_CHECK(params && center && rows && n >= 0, KI_E_ARG, "bad argument");
-    const ki::ProxySpec px = ki::make_proxy(*params, c.dom);
+    KI_HIP(hipSetDevice(c.device));
+    const ki::ProxySpec px = ki::device_proxy(c, *params);
     const int P = 2 * c.sol * px.npts;
     *rows = P;
     if (!out || n == 0) return;

@@ -40,45 +40,30 @@ __device__ __forceinline__ void load_pt(const DevGeom& g, int p, Pt& o) {
 __device__ __forceinline__ void proxy_pt(const ProxySpec& px, int dom,
                                          const double* c, double R, int q,
                                          Pt& o) {
-  const double PI = 3.14159265358979323846;
+  const double* u = px.tab + 4 * (int64_t)q;
+  const double u0 = u[0], u1 = u[1], u2 = u[2], w1 = u[3];
   o.c = 0.0;
-  if (dom == 2) {
-    const double ang = 2 * PI * q * (1.0 / px.npts);
-    const double ca = cos(ang), sa = sin(ang);
-    o.x[0] = c[0] + R * ca;
-    o.x[1] = c[1] + R * sa;
-    o.x[2] = 0;
-    o.n[0] = ca;
-    o.n[1] = sa;
-    o.n[2] = 0;
-    o.w = 2.0 * PI * R / px.npts;
-    return;
-  }
-  if (px.kind == 0) {
-    const int i = q / px.nphi, j = q % px.nphi;
-    const double th = 2 * PI * i * (1.0 / px.ntheta);
-    const double ph = px.gl_phi[j];
-    const double sp = sin(ph), cp = cos(ph), ct = cos(th), st = sin(th);
-    o.x[0] = c[0] + R * sp * ct;
-    o.x[1] = c[1] + R * sp * st;
-    o.x[2] = c[2] + R * cp;
-    o.n[0] = sp * ct;
-    o.n[1] = sp * st;
-    o.n[2] = cp;
-    o.w = R * R * (px.gl_w[j] * sp) * (2 * PI / px.ntheta);
-  } else {
-    const double z = 1.0 - (2.0 * q + 1.0) / px.npts;
-    const double rho = sqrt(fmax(0.0, 1.0 - z * z));
-    const double th = PI * (3.0 - sqrt(5.0)) * q;
-    const double u0 = rho * cos(th), u1 = rho * sin(th);
-    o.x[0] = c[0] + R * u0;
-    o.x[1] = c[1] + R * u1;
-    o.x[2] = c[2] + R * z;
-    o.n[0] = u0;
-    o.n[1] = u1;
-    o.n[2] = z;
-    o.w = 4.0 * PI * R * R / px.npts;
-  }
+  o.x[0] = c[0] + R * u0;
+  o.x[1] = c[1] + R * u1;
+  o.x[2] = dom == 2 ? 0.0 : c[2] + R * u2;
+  o.n[0] = u0;
+  o.n[1] = u1;
+  o.n[2] = u2;
+  o.w = (dom == 2 ? R : R * R) * w1;
+}
+
+// 1/sqrt(x): FP64 divides by the correctly rounded square root; FP32 takes
+// the hardware reciprocal square root (v_rsq_f32, within 1 ulp), which is
+// all the FP32 form's entries need.
+__device__ __forceinline__ double inv_sqrt(double x) { return 1.0 / sqrt(x); }
+__device__ __forceinline__ float inv_sqrt(float x) {
+  return __builtin_amdgcn_rsqf(x);
+}
+
+// Component c of a 3-vector by selects, not by a runtime array index (which
+// would put the point structs in private memory).
+__device__ __forceinline__ double comp3(const double (&v)[3], int c) {
+  return c == 0 ? v[0] : (c == 1 ? v[1] : v[2]);
 }
 
 // K(target, source) for one (row component a, column component b) pair.
@@ -99,34 +84,35 @@ __device__ __forceinline__ T entry(const DevGeom& g, const Pt& t, int a,
     if (zero) {
       const int isd = (a + b + 1) % 2;
       return (T)(-0.5 * isd + (1 - 2 * isd) * 0.5 * s.c * s.w * (1 / PI) *
-                                  s.n[(a + 1) % 2] * s.n[(b + 1) % 2] +
-                 s.w * t.n[a] * s.n[b]);
+                                  comp3(s.n, (a + 1) % 2) *
+                                  comp3(s.n, (b + 1) % 2) +
+                 s.w * comp3(t.n, a) * comp3(s.n, b));
     }
     const T x0 = (T)r0, x1 = (T)r1;
     const T rr = x0 * x0 + x1 * x1;
     const T dot = x0 * (T)s.n[0] + x1 * (T)s.n[1];
     const T ra = a == 0 ? x0 : x1, rb = b == 0 ? x0 : x1;
     return (T)s.w * (T)(1 / PI) * dot / (rr * rr) * ra * rb +
-           (T)s.w * (T)t.n[a] * (T)s.n[b];
+           (T)s.w * (T)comp3(t.n, a) * (T)comp3(s.n, b);
   }
   if (g.sol == 1) {
     if (zero) return (T)g.diag[sidx];
     const T x0 = (T)r0, x1 = (T)r1, x2 = (T)r2;
     const T rr = x0 * x0 + x1 * x1 + x2 * x2;
     const T dot = x0 * (T)s.n[0] + x1 * (T)s.n[1] + x2 * (T)s.n[2];
-    const T ir = (T)1 / sqrt(rr);
+    const T ir = inv_sqrt(rr);
     return -(T)s.w * (T)(1.0 / (4 * PI)) * dot * (ir * ir * ir);
   }
   if (zero) return (T)g.diag[(int64_t)sidx * 9 + a * 3 + b];
   const T x0 = (T)r0, x1 = (T)r1, x2 = (T)r2;
   const T rr = x0 * x0 + x1 * x1 + x2 * x2;
   const T dot = x0 * (T)s.n[0] + x1 * (T)s.n[1] + x2 * (T)s.n[2];
-  const T ir = (T)1 / sqrt(rr);
+  const T ir = inv_sqrt(rr);
   const T ir2 = ir * ir;
   const T ra = a == 0 ? x0 : (a == 1 ? x1 : x2);
   const T rb = b == 0 ? x0 : (b == 1 ? x1 : x2);
   return (T)s.w * (T)(-3.0 / (4.0 * PI)) * dot * (ir2 * ir2 * ir) * ra * rb +
-         (T)s.w * (T)t.n[a] * (T)s.n[b];
+         (T)s.w * (T)comp3(t.n, a) * (T)comp3(s.n, b);
 }
 
 // One workgroup fills a tile of FR rows x FC columns of one block: each
@@ -135,17 +121,192 @@ __device__ __forceinline__ T entry(const DevGeom& g, const Pt& t, int a,
 // FC entries; for every column the 256 threads store 256 consecutive rows.
 constexpr int FR = 256, FC = 16;
 
+// Row sketch.  Position p of the sketch order holds block row
+// sketch_perm(p), a keyed bijection of [0, m): four alternating xor rounds
+// on the two halves of a ceil(log2 m)-bit word, whose round function is the
+// splitmix64 output mix (Steele, Lea & Flood, 2014), with cycle walking back
+// into [0, m).  Bucket r of Y adds row sketch_perm(p), negated when
+// sketch_negative(p), over p = r, r + B, r + 2B, ... < m, so every row lands
+// in exactly one of the B buckets and the buckets differ in size by at most
+// one: a random balanced row partition with random signs, for which
+// E[Y^T Y] = A^T A.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ uint32_t perm_pass(uint32_t x, int hr, uint32_t ml,
+                                              uint32_t mr, uint64_t k) {
+  uint32_t L = x >> hr, R = x & mr;
+#pragma unroll
+  for (int rd = 0; rd < 4; rd++) {
+    const uint64_t v = (rd & 1) ? L : R;
+    const uint64_t f =
+        mix64((k | ((uint64_t)rd << 24) | v) + 0x9E3779B97F4A7C15ull);
+    if (rd & 1)
+      R ^= (uint32_t)f & mr;
+    else
+      L ^= (uint32_t)f & ml;
+  }
+  return (L << hr) | R;
+}
+
+__device__ __forceinline__ bool sketch_negative(uint64_t k, uint32_t p) {
+  return (mix64((k | p) + 0xD1B54A32D192ED03ull) >> 63) != 0;
+}
+
+// The row of block row i (segment lookup): point data, component, and
+// whether the row point is the target of K (K rows, box->proxy rows) or its
+// source (transposed rows, proxy->box rows).
 template <typename T>
-__global__ void __launch_bounds__(FR)
-fill_kernel(DevGeom g, ProxySpec px, const FillDesc* __restrict__ desc,
-            const int* __restrict__ idx, T* __restrict__ out) {
-  const FillDesc& d = desc[blockIdx.z];
-  const int c0 = blockIdx.x * FC;
-  if (c0 >= d.ncols) return;
+__device__ __forceinline__ void resolve_row(const DevGeom& g,
+                                            const ProxySpec& px,
+                                            const FillDesc& d,
+                                            const int* __restrict__ idx, int i,
+                                            Pt& rp, int& rcomp, int& rpt,
+                                            bool& row_tgt, bool& self_ok) {
+  const int sol = g.sol;
+  int sg = 0, base = 0;
+  while (i >= base + d.seg_rows[sg]) base += d.seg_rows[sg++];
+  const int kind = d.seg_kind[sg], ii = i - base, nr = d.seg_rows[sg];
+  rpt = 0;
+  if (kind == SEG_K || kind == SEG_KT) {
+    const int rd = idx[d.seg_idx[sg] + ii];
+    rpt = rd / sol;
+    rcomp = rd % sol;
+    load_pt(g, rpt, rp);
+    row_tgt = kind == SEG_K;
+    self_ok = true;
+  } else {
+    const int half = nr / 2;
+    const int q = ii < half ? ii : ii - half;
+    proxy_pt(px, g.dom, d.center, d.radius, q / sol, rp);
+    rcomp = q % sol;
+    row_tgt = ii < half;
+    self_ok = false;
+  }
+}
+
+// Sketch row table (RowTab): one thread per sketch position p of a block
+// resolves ID row sketch_perm(p) and its sign.  The random gathers happen
+// here, once per row; the fill then streams the table.
+__global__ void __launch_bounds__(256)
+sketch_rows_kernel(DevGeom g, ProxySpec px, const FillDesc* __restrict__ desc,
+                   const int* __restrict__ idx, RowTab tab) {
+  const FillDesc& d = desc[blockIdx.y];
+  if (d.buckets <= 0) return;
   int m = 0;
   for (int sg = 0; sg < d.nseg; sg++) m += d.seg_rows[sg];
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= m) return;
+  const int bits = max(2, 32 - __clz(m - 1));
+  const int hr = bits - (bits >> 1);
+  const uint32_t ml = (1u << (bits >> 1)) - 1, mr = (1u << hr) - 1;
+  const uint64_t k = (uint64_t)d.key << 32;
+  uint32_t i = perm_pass((uint32_t)p, hr, ml, mr, k);
+  // the orbit of p returns to p < m: the walk ends within 2^bits passes
+  for (uint32_t w = 0; i >= (uint32_t)m && w < (1u << bits); w++)
+    i = perm_pass(i, hr, ml, mr, k);
+  if (i >= (uint32_t)m) i = p;  // unreachable for a bijection
+  Pt rp;
+  int rcomp, rpt;
+  bool row_tgt, self_ok;
+  resolve_row<double>(g, px, d, idx, (int)i, rp, rcomp, rpt, row_tgt,
+                      self_ok);
+  const int64_t t = d.tab_off + p, S = tab.stride;
+  for (int q = 0; q < 3; q++) {
+    tab.d[q * S + t] = rp.x[q];
+    tab.d[(3 + q) * S + t] = rp.n[q];
+  }
+  tab.d[6 * S + t] = rp.w;
+  tab.d[7 * S + t] = rp.c;
+  tab.i[t] = rcomp | (row_tgt ? 1 << 8 : 0) | (self_ok ? 1 << 9 : 0) |
+             (sketch_negative(k, (uint32_t)p) ? 1 << 10 : 0);
+  tab.i[S + t] = rpt;
+}
+
+// Sketch form of fill_kernel: the thread owns bucket r and walks its rows
+// p = r, r + B, ... in the row table (consecutive buckets read consecutive
+// table entries); one entry() call per (row, column) with target and source
+// selected per lane, since neighbouring buckets hold rows of different
+// segments.
+template <typename T>
+__device__ __forceinline__ void sketch_tile(const DevGeom& g, const FillDesc& d,
+                                            const RowTab& tab, const Pt* cpts,
+                                            const int* cdofs, int m, int r,
+                                            int c0, int nc,
+                                            T* __restrict__ out) {
+  const int sol = g.sol;
+  const int B = d.buckets;
+  const int64_t S = tab.stride;
+  // bucket sums in the output type: a bucket adds a handful of rows, and
+  // FP32 sums of FP32 entries keep the entries' own accuracy
+  T acc[FC];
+#pragma unroll
+  for (int jj = 0; jj < FC; jj++) acc[jj] = (T)0;
+  for (int p = r; p < m; p += B) {
+    // compiler-only fence: the column points are re-read from LDS in every
+    // row instead of being hoisted into registers for all FC columns (which
+    // takes the whole register file and leaves one wave per SIMD)
+    __asm__ volatile("" ::: "memory");
+    const int64_t t = d.tab_off + p;
+    Pt rp;
+    for (int q = 0; q < 3; q++) {
+      rp.x[q] = tab.d[q * S + t];
+      rp.n[q] = tab.d[(3 + q) * S + t];
+    }
+    rp.w = tab.d[6 * S + t];
+    rp.c = g.dom == 2 ? tab.d[7 * S + t] : 0.0;
+    const int info = tab.i[t], rpt = tab.i[S + t];
+    const int rcomp = info & 255;
+    const bool row_tgt = (info >> 8) & 1, self_ok = (info >> 9) & 1,
+               neg = (info >> 10) & 1;
+#pragma unroll
+    for (int jj = 0; jj < FC; jj++) {
+      if (jj < nc) {
+        const Pt cp = cpts[jj];
+        const int cd = cdofs[jj];
+        const int cpi = cd / sol, cpar = cd % sol;
+        Pt tp, sp;
+        for (int q = 0; q < 3; q++) {
+          tp.x[q] = row_tgt ? rp.x[q] : cp.x[q];
+          tp.n[q] = row_tgt ? rp.n[q] : cp.n[q];
+          sp.x[q] = row_tgt ? cp.x[q] : rp.x[q];
+          sp.n[q] = row_tgt ? cp.n[q] : rp.n[q];
+        }
+        tp.w = row_tgt ? rp.w : cp.w;
+        tp.c = row_tgt ? rp.c : cp.c;
+        sp.w = row_tgt ? cp.w : rp.w;
+        sp.c = row_tgt ? cp.c : rp.c;
+        const T v = entry<T>(g, tp, row_tgt ? rcomp : cpar, sp,
+                             row_tgt ? cpar : rcomp, row_tgt ? cpi : rpt,
+                             self_ok);
+        acc[jj] += neg ? -v : v;
+      }
+    }
+  }
+  T* row = out + d.out_off + r;
+#pragma unroll
+  for (int jj = 0; jj < FC; jj++)
+    if (jj < nc) row[(int64_t)d.ld * (c0 + jj)] = acc[jj];
+}
+
+// SKETCH: the sketched blocks of the launch (buckets > 0) only, else the
+// others; two instantiations, so that the plain fill keeps its own register
+// budget.
+template <typename T, bool SKETCH>
+__global__ void __launch_bounds__(FR)
+fill_kernel(DevGeom g, ProxySpec px, const FillDesc* __restrict__ desc,
+            const int* __restrict__ idx, T* __restrict__ out, RowTab tab) {
+  const FillDesc& d = desc[blockIdx.z];
+  const int c0 = blockIdx.x * FC;
+  if (c0 >= d.ncols || SKETCH != (d.buckets > 0)) return;
+  int m = 0;
+  for (int sg = 0; sg < d.nseg; sg++) m += d.seg_rows[sg];
+  const int rows_out = d.buckets > 0 ? d.buckets : m;
   const int i = blockIdx.y * FR + threadIdx.x;
-  if ((int)blockIdx.y * FR >= m) return;
+  if ((int)blockIdx.y * FR >= rows_out) return;
   const int sol = g.sol;
   const int nc = min(FC, d.ncols - c0);
   __shared__ Pt cpts[FC];
@@ -156,7 +317,11 @@ fill_kernel(DevGeom g, ProxySpec px, const FillDesc* __restrict__ desc,
     load_pt(g, cd / sol, cpts[threadIdx.x]);
   }
   __syncthreads();
-  if (i >= m) return;
+  if (i >= rows_out) return;
+  if constexpr (SKETCH) {
+    sketch_tile<T>(g, d, tab, cpts, cdofs, m, i, c0, nc, out);
+    return;
+  }
   int sg = 0, base = 0;
   while (i >= base + d.seg_rows[sg]) base += d.seg_rows[sg++];
   const int kind = d.seg_kind[sg], ii = i - base, nr = d.seg_rows[sg];
@@ -179,17 +344,17 @@ fill_kernel(DevGeom g, ProxySpec px, const FillDesc* __restrict__ desc,
   for (int jj = 0; jj < nc; jj++) {
     const int cd = cdofs[jj];
     const int cpi = cd / sol, cpar = cd % sol;
+    Pt cp = cpts[jj];  // register copy: entry() selects its components
     T v;
     if (kind == SEG_K) {
-      v = entry<T>(g, rp, rcomp, cpts[jj], cpar, cpi, true);
+      v = entry<T>(g, rp, rcomp, cp, cpar, cpi, true);
     } else if (kind == SEG_KT) {
-      v = entry<T>(g, cpts[jj], cpar, rp, rcomp, rpt, true);
+      v = entry<T>(g, cp, cpar, rp, rcomp, rpt, true);
     } else if (to_proxy) {  // box -> proxy: target proxy, source column DOF
-      Pt src = cpts[jj];
-      src.c = 0.0;
-      v = entry<T>(g, rp, rcomp, src, cpar, cpi, false);
+      cp.c = 0.0;
+      v = entry<T>(g, rp, rcomp, cp, cpar, cpi, false);
     } else {                // proxy -> box: target column DOF, source proxy
-      v = entry<T>(g, cpts[jj], cpar, rp, rcomp, 0, false);
+      v = entry<T>(g, cp, cpar, rp, rcomp, 0, false);
     }
     row[(int64_t)d.ld * (c0 + jj)] = v;
   }
@@ -197,24 +362,41 @@ fill_kernel(DevGeom g, ProxySpec px, const FillDesc* __restrict__ desc,
 
 }  // namespace
 
+void launch_sketch_rows(const DevGeom& g, const ProxySpec& px,
+                        const FillDesc* d_desc, int ndesc, int max_rows,
+                        const int* d_idx, RowTab tab, hipStream_t s) {
+  if (ndesc == 0 || max_rows == 0) return;
+  KI_CHECK(ndesc <= 65535, 2, "too many blocks in one sketch launch");
+  KI_CHECK(tab.d && tab.i, 2, "sketch row table missing");
+  dim3 grid((max_rows + 255) / 256, ndesc);
+  hipLaunchKernelGGL(sketch_rows_kernel, grid, dim3(256), 0, s, g, px, d_desc,
+                     d_idx, tab);
+  KI_HIP(hipGetLastError());
+}
+
 template <typename T>
 void launch_fill(const DevGeom& g, const ProxySpec& px, const FillDesc* d_desc,
                  int ndesc, int max_cols, int max_rows, const int* d_idx,
-                 T* out, hipStream_t s) {
+                 T* out, hipStream_t s, RowTab tab) {
   if (ndesc == 0 || max_cols == 0 || max_rows == 0) return;
   KI_CHECK(ndesc <= 65535, 2, "too many blocks in one fill launch");
   KI_CHECK((max_rows + FR - 1) / FR <= 65535, 2, "block too tall to fill");
   dim3 grid((max_cols + FC - 1) / FC, (max_rows + FR - 1) / FR, ndesc);
-  hipLaunchKernelGGL(fill_kernel<T>, grid, dim3(FR), 0, s, g, px, d_desc,
-                     d_idx, out);
+  hipLaunchKernelGGL((fill_kernel<T, false>), grid, dim3(FR), 0, s, g, px,
+                     d_desc, d_idx, out, tab);
   KI_HIP(hipGetLastError());
+  if (tab.d) {
+    hipLaunchKernelGGL((fill_kernel<T, true>), grid, dim3(FR), 0, s, g, px,
+                       d_desc, d_idx, out, tab);
+    KI_HIP(hipGetLastError());
+  }
 }
 
 template void launch_fill<float>(const DevGeom&, const ProxySpec&,
                                  const FillDesc*, int, int, int, const int*,
-                                 float*, hipStream_t);
+                                 float*, hipStream_t, RowTab);
 template void launch_fill<double>(const DevGeom&, const ProxySpec&,
                                   const FillDesc*, int, int, int, const int*,
-                                  double*, hipStream_t);
+                                  double*, hipStream_t, RowTab);
 
 }  // namespace ki

@@ -112,6 +112,10 @@ struct ProxySpec {
   int kind;        // 0 = 2D equispaced circle / 3D GL tensor, 1 = Fibonacci
   int ntheta, nphi;
   double gl_phi[64], gl_w[64];  // GL nodes/weights on [0, pi] (3D tensor)
+  // device table, 4 doubles per proxy point: the unit-surface point u (the
+  // outward normal) and weight w1; a box's proxy point q is centre + R u,
+  // normal u, weight R^(dom-1) w1
+  const double* tab;
 };
 
 // One generated block: up to three row segments stacked vertically, all with
@@ -127,13 +131,36 @@ struct FillDesc {
   int ld;
   double center[3];        // proxy centre (SEG_PROXY)
   double radius;           // proxy radius (SEG_PROXY)
+  // buckets > 0: write the row sketch Y = S A (buckets rows, ld = buckets)
+  // instead of A: bucket r = sum over p = r, r + buckets, ... < m of
+  // +-A[sketch_perm(key, m, p)] (kernels_fill.hip).
+  int buckets;
+  uint32_t key;            // level << 20 | index of the box in its level
+  int64_t tab_off;         // first row of the box in the sketch row table
 };
 
-// max_cols / max_rows: the largest block of the launch.
+// Sketch row table: the rows of every sketched block of a launch in sketch
+// order, structure of arrays with `stride` entries per field.  Doubles:
+// point x0 x1 x2, normal n0 n1 n2, weight, curvature; ints: component |
+// row-is-target << 8 | self-term allowed << 9 | negated << 10, and the row's
+// point index (for the 3D self term).
+struct RowTab {
+  double* d = nullptr;
+  int* i = nullptr;
+  int64_t stride = 0;
+};
+
+// Writes the row table of the sketched blocks (buckets > 0) of a launch.
+void launch_sketch_rows(const DevGeom& g, const ProxySpec& px,
+                        const FillDesc* d_desc, int ndesc, int max_rows,
+                        const int* d_idx, RowTab tab, hipStream_t s);
+
+// max_cols / max_rows: the largest block of the launch (rows written).
+// Sketched blocks read their rows from `tab` (launch_sketch_rows first).
 template <typename T>
 void launch_fill(const DevGeom& g, const ProxySpec& px, const FillDesc* d_desc,
                  int ndesc, int max_cols, int max_rows, const int* d_idx,
-                 T* out, hipStream_t s);
+                 T* out, hipStream_t s, RowTab tab = RowTab{});
 
 // Scatter-add jobs: dst[map[i] + ld*map[j]] += sign * src[i + n*j] for an
 // n x n source; with acc, entries whose mapped row and column are both

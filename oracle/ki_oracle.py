@@ -18,6 +18,11 @@ factorization and solve in plain FP64 (paths relative to the reference repo):
                                    LAPACK, the same routines)
 * decouple, skeletonize, solve     skel_factorization/skel_factorization.cpp:
                                    36-58, 91-127, 173-310, 555-778
+* row sketch of an ID matrix       no counterpart in the reference: the FP32
+                                   form's option of running the ID on
+                                   Y = S A (a random balanced row partition
+                                   with random signs); restated here so the
+                                   GPU sketch can be checked entry by entry
 
 The reference's compile-time constants (MAX_LEAF_DOFS 64, MIN_DOFS_TO_COMPRESS
 64, NUM_PROXY_POINTS 128, RADIUS_RATIO 2, the 8x4 3D proxy rule) are the
@@ -29,7 +34,10 @@ invariants and a hand-checked 2D example; kernel entries by closed forms
 ``A[:, p] = Q R`` identity and the ``||A_r - A_s T|| <= tol |R00|`` bound; the
 factorization by reproducing the dense matrix's solution at small N (dense
 LU) to round-off when the tolerance is tight; end to end by the analytic
-interior fields.
+interior fields.  The row sketch: its row map is a bijection; with one row per
+bucket it is a signed permutation (Y^T Y = A^T A exactly); averaged over keys
+Y^T Y tends to A^T A; and the ID of a sketched ID matrix interpolates the
+unsketched one within a small multiple of the tolerance.
 """
 from __future__ import annotations
 
@@ -52,6 +60,7 @@ class Params:
     proxy_points: int = 128      # kernel.h:13 (2D); 3D: n_theta*n_phi
     proxy_ratio: float = 2.0     # kernel.h:14, times sqrt(d) * half side
     proxy3d_fibonacci: bool = False  # False: reference 8x4 GL tensor rule
+    id_sketch: int = 0           # > 0: ID on a sketch with id_sketch * n rows
 
 
 # ---------------------------------------------------------------------------
@@ -511,6 +520,77 @@ def interpolative_decomposition(A: np.ndarray, tol: float):
 
 
 # ---------------------------------------------------------------------------
+# Row sketch of an ID matrix (no counterpart in the reference)
+# ---------------------------------------------------------------------------
+#
+# Y = S A with `buckets` rows.  Position p of the sketch order holds ID row
+# perm(p), and bucket r sums sign(p) * A[perm(p)] over p = r, r + buckets, ...
+# perm is a keyed bijection of [0, m): four alternating xor rounds on the two
+# halves of a ceil(log2 m)-bit word (a Feistel-type network whose round
+# function is the splitmix64 output mix of Steele, Lea & Flood, 2014), with
+# cycle walking back into [0, m).  key = level << 20 | index within level.
+
+def _mix64(z):
+    z = np.asarray(z, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def sketch_perm_pass(x, bits: int, key: int):
+    """One pass of the keyed bijection of [0, 2^bits)."""
+    hl = bits >> 1
+    hr = bits - hl
+    ml, mr = np.uint64((1 << hl) - 1), np.uint64((1 << hr) - 1)
+    x = np.asarray(x, dtype=np.uint64)
+    L, R = x >> np.uint64(hr), x & mr
+    k = np.uint64(key) << np.uint64(32)
+    for rd in range(4):
+        v = L if rd & 1 else R
+        with np.errstate(over="ignore"):
+            f = _mix64((k | (np.uint64(rd) << np.uint64(24)) | v) +
+                       np.uint64(0x9E3779B97F4A7C15))
+        if rd & 1:
+            R = R ^ (f & mr)
+        else:
+            L = L ^ (f & ml)
+    return (L << np.uint64(hr)) | R
+
+
+def sketch_perm(m: int, key: int) -> np.ndarray:
+    """perm(p) for p in [0, m): the ID row held at sketch position p."""
+    bits = max(2, int(m - 1).bit_length())
+    out = sketch_perm_pass(np.arange(m, dtype=np.uint64), bits, key)
+    todo = out >= np.uint64(m)
+    while todo.any():
+        out[todo] = sketch_perm_pass(out[todo], bits, key)
+        todo = out >= np.uint64(m)
+    return out.astype(np.int64)
+
+
+def sketch_signs(m: int, key: int) -> np.ndarray:
+    z = (np.uint64(key) << np.uint64(32)) | np.arange(m, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        h = _mix64(z + np.uint64(0xD1B54A32D192ED03))
+    return np.where((h >> np.uint64(63)) != 0, -1.0, 1.0)
+
+
+def sketch_rows(A: np.ndarray, buckets: int, key: int) -> np.ndarray:
+    """Y = S A (buckets x n)."""
+    m = A.shape[0]
+    perm = sketch_perm(m, key)
+    sg = sketch_signs(m, key)
+    Y = np.zeros((buckets, A.shape[1]))
+    np.add.at(Y, np.arange(m) % buckets, sg[:, None] * A[perm])
+    return Y
+
+
+def sketch_key(node) -> int:
+    return (node.level << 20) | node.index
+
+
+# ---------------------------------------------------------------------------
 # Factorization and solve (skel_factorization.cpp)
 # ---------------------------------------------------------------------------
 
@@ -547,6 +627,9 @@ class SkelFactorization:
         A = kern.id_mat(tree, node)
         if A.shape[0] == 0:
             return 0
+        bk = self.p.id_sketch * len(node.active)
+        if bk > 0 and A.shape[0] >= 2 * bk:
+            A = sketch_rows(A, bk, sketch_key(node))
         k, p, T = interpolative_decomposition(A, self.p.id_tol)
         if k == 0:
             return 0

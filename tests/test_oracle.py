@@ -308,6 +308,102 @@ def test_id_full_rank_returns_zero(O):
 
 
 # ---------------------------------------------------------------------------
+# Row sketch of an ID matrix
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("m", [1, 2, 3, 64, 65, 1000, 4096, 4097, 17246])
+def test_sketch_perm_is_a_bijection(O, m):
+    for key in (0, (3 << 20) | 17):
+        p = O.sketch_perm(m, key)
+        assert np.array_equal(np.sort(p), np.arange(m))
+    # keyed: two boxes get different orders
+    if m >= 64:
+        assert not np.array_equal(O.sketch_perm(m, 1), O.sketch_perm(m, 2))
+
+
+def test_sketch_with_one_row_per_bucket_is_a_signed_permutation(O):
+    rng = np.random.default_rng(4)
+    A = rng.normal(size=(300, 40))
+    Y = O.sketch_rows(A, 300, key=9)
+    # every row of A once, up to sign: the Gram matrix is unchanged
+    assert np.allclose(Y.T @ Y, A.T @ A, rtol=0, atol=1e-12 * 300)
+    ref ={tuple(np.round(np.abs(r), 12)) for r in A}
+    got = {tuple(np.round(np.abs(r), 12)) for r in Y}
+    assert ref == got
+    # and the ID of the sketch is the ID of A
+    U = rng.normal(size=(300, 8)) @ rng.normal(size=(8, 40))
+    k0, p0, _ = O.interpolative_decomposition(U, 1e-9)
+    k1, p1, _ = O.interpolative_decomposition(O.sketch_rows(U, 300, 5), 1e-9)
+    assert k0 == k1 == 8 and set(p0[:k0]) == set(p1[:k1])
+
+
+def test_sketch_gram_is_unbiased_and_the_signs_matter(O):
+    """E[Y^T Y] = A^T A over keys.  With positive entries, bucket sums
+    without the random signs would add the in-bucket cross terms (here about
+    as large as A^T A itself), so a dropped sign fails this."""
+    rng = np.random.default_rng(6)
+    A = rng.uniform(0.5, 1.5, size=(60, 5))
+    G0 = A.T @ A
+    acc = np.zeros_like(G0)
+    nk = 3000
+    for key in range(nk):
+        Y = O.sketch_rows(A, 12, key)
+        acc += Y.T @ Y
+    rel = np.linalg.norm(acc / nk - G0) / np.linalg.norm(G0)
+    assert rel < 0.05, rel
+    perm = O.sketch_perm(60, 0)
+    Yu = np.zeros((12, 5))
+    np.add.at(Yu, np.arange(60) % 12, A[perm])
+    assert np.linalg.norm(Yu.T @ Yu - G0) / np.linalg.norm(G0) > 1.0
+
+
+def test_sketched_id_interpolates_stokes3d_id_matrices(O):
+    """The ID matrices of a 3D Stokes ellipsoid factorization: the ID of each
+    one's 4n-row sketch keeps the rank within 5% and interpolates A within a
+    small multiple of the tolerance; the sketched factorization solves as
+    well as the exact one."""
+    b = G.ellipsoid(1024)
+    pb = G.make_problem(b, "stokes", 1e-6, 256, 256, 1.5, 6, 4)
+    prm = O.Params(id_tol=1e-6, max_leaf_dofs=256, proxy_points=256,
+                   proxy_ratio=1.5, proxy3d_fibonacci=True)
+    seen = []
+
+    class Recording(O.SkelFactorization):
+        def id_compress(self, kern, tree, node):
+            A = kern.id_mat(tree, node)
+            if A.shape[0] >= 8 * A.shape[1] > 0:
+                seen.append((A, O.sketch_key(node)))
+            return super().id_compress(kern, tree, node)
+
+    kern = O.Kernel(b, O.STOKES, prm)
+    tree = O.Tree(b.points, b.dom, kern.sol, 256)
+    Recording(prm).skeletonize(kern, tree)
+    compressed = 0
+    for A, key in seen:
+        n = A.shape[1]
+        k0, p0, T0 = O.interpolative_decomposition(A, prm.id_tol)
+        k1, p1, T1 = O.interpolative_decomposition(
+            O.sketch_rows(A, 4 * n, key), prm.id_tol)
+        if k0 == 0:         # incompressible at tol: so is the sketch
+            assert k1 == 0 or k1 >= 0.9 * n, (k0, k1)
+            continue
+        assert abs(k1 - k0) <= 0.05 * k0 + 2, (k0, k1)
+        compressed += 1
+        s, r = p1[:k1], p1[k1:]
+        err = np.linalg.norm(A[:, r] - A[:, s] @ T1, 2)
+        assert err <= 20 * prm.id_tol * np.linalg.norm(A, 2), err
+    assert compressed >= 2, (len(seen), compressed)
+    K = kern.dense()
+    res = {}
+    for sk in (0, 4):
+        import dataclasses
+        p2 = dataclasses.replace(prm, id_sketch=sk)
+        _, t2, f2, mu = O.factor_and_solve(b, O.STOKES, p2, pb.rhs)
+        res[sk] = np.linalg.norm(K @ mu - pb.rhs) / np.linalg.norm(pb.rhs)
+    assert res[4] < 10 * res[0] + 1e-8, res
+
+
+# ---------------------------------------------------------------------------
 # Factorization and solve
 # ---------------------------------------------------------------------------
 

@@ -146,14 +146,15 @@ def test_matvec_matches_dense(name):
     assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
-def _oracle_factor(name):
+def _oracle_factor(name, id_sketch=0):
+    import dataclasses
     import ki_oracle as O
-    key = name + "/fac"
+    key = name + "/fac/%d" % id_sketch
     if key in _cache:
         return _cache[key]
     b, pde, kw, leaf, prm, ctx, okern, otree = setup(name)
     tree = O.Tree(b.points, b.dom, okern.sol, leaf)
-    fac = O.SkelFactorization(prm)
+    fac = O.SkelFactorization(dataclasses.replace(prm, id_sketch=id_sketch))
     fac.skeletonize(okern, tree)
     _cache[key] = (tree, fac)
     return _cache[key]
@@ -176,15 +177,10 @@ def _gpu_id_matrix(ctx, okern, tree, l, i):
     return okern.id_mat(NS(levels=levels), me), me.active
 
 
-@pytest.mark.parametrize("name", CASES)
-def test_skeletons_match_oracle_fp64(name):
-    """Plain FP64 GPU path: same compressed boxes, ranks and skeleton sets as
-    the oracle's dgeqp3-based ID (sets compared, pivot order may differ only
-    through exact ties)."""
+def _check_skeletons(name, ctx, tree, fac, kw, id_sketch=0):
+    """Same compressed boxes, ranks and skeleton sets as the oracle (sets
+    compared; a different set is accepted only as a pivot tie)."""
     import ki_oracle as O
-    b, pde, kw, leaf, prm, ctx, okern, otree = setup(name)
-    ctx.skeletonize(ki.default_params(precision=64, **kw))
-    tree, fac = _oracle_factor(name)
     total = same = tied = 0
     for l, lv in enumerate(tree.levels):
         for i, nd in enumerate(lv):
@@ -200,11 +196,16 @@ def test_skeletons_match_oracle_fp64(name):
                 continue
             # A different skeleton is accepted only as a pivot tie: the ID
             # matrix is rebuilt from the GPU's own lists (a tie below changes
-            # this box's active set), the oracle's dgeqp3 ID is run on it,
-            # and the GPU skeleton must interpolate the redundant columns as
-            # well as that ID does.
-            A, act = _gpu_id_matrix(ctx, okern, tree, l, i)
-            k_o, p_o, _ = O.interpolative_decomposition(A, kw["id_tol"])
+            # this box's active set), the oracle's dgeqp3 ID is run on it
+            # (on its sketch when the factorization sketched), and the GPU
+            # skeleton must interpolate the redundant columns as well as
+            # that ID does.
+            A, act = _gpu_id_matrix(ctx, okern_of(name), tree, l, i)
+            bk = id_sketch * len(act)
+            Y = A
+            if bk > 0 and A.shape[0] >= 2 * bk:
+                Y = O.sketch_rows(A, bk, (l << 20) | i)
+            k_o, p_o, _ = O.interpolative_decomposition(Y, kw["id_tol"])
             assert k_o == len(sk), (l, i, k_o, len(sk))
             pos = {v: q for q, v in enumerate(act)}
             s_idx = [pos[v] for v in sk]
@@ -217,10 +218,46 @@ def test_skeletons_match_oracle_fp64(name):
             assert err <= max(10 * err_o, 10 * kw["id_tol"] *
                               np.linalg.norm(A, 2)), (l, i, err, err_o)
             tied += 1
-    print("[skel] %s: %d/%d boxes with identical skeleton sets, %d ties"
-          % (name, same, total, tied))
+    print("[skel] %s sketch %d: %d/%d boxes with identical skeleton sets, "
+          "%d ties" % (name, id_sketch, same, total, tied))
     assert same + tied == total
     assert ctx.stats()["top_size"] == len(fac.allskel)
+
+
+def okern_of(name):
+    return setup(name)[6]
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_skeletons_match_oracle_fp64(name):
+    """Plain FP64 GPU path: same compressed boxes, ranks and skeleton sets as
+    the oracle's dgeqp3-based ID (sets compared, pivot order may differ only
+    through exact ties)."""
+    b, pde, kw, leaf, prm, ctx, okern, otree = setup(name)
+    ctx.skeletonize(ki.default_params(precision=64, **kw))
+    tree, fac = _oracle_factor(name)
+    _check_skeletons(name, ctx, tree, fac, kw)
+
+
+@pytest.mark.parametrize("name", ["lap3d_sphere", "stokes3d_ellipsoid"])
+def test_sketched_skeletons_match_oracle_fp64(name):
+    """FP64 with the row sketch forced on (4 rows per column, Gram form): the
+    GPU's fused sketch fill, Gram and pivoted Cholesky give the skeletons of
+    the oracle's dgeqp3 ID of the same sketch (same permutation and signs),
+    and the refined solution still meets 1e-10."""
+    b, pde, kw, leaf, prm, ctx, okern, otree = setup(name)
+    ctx.skeletonize(ki.default_params(precision=64, id_method=2, id_sketch=4,
+                                      **kw))
+    tree, fac = _oracle_factor(name, id_sketch=4)
+    n_sk = sum(1 for l, lv in enumerate(tree.levels) for nd in lv
+               if nd.compressed and 2 * 4 * len(nd.skel + nd.redundant) <=
+               _gpu_id_matrix(ctx, okern, tree, l, nd.index)[0].shape[0])
+    print("[sketch] %s: %d sketched boxes" % (name, n_sk))
+    assert n_sk > 0
+    _check_skeletons(name, ctx, tree, fac, kw, id_sketch=4)
+    rhs = np.random.default_rng(3).normal(size=ctx.ndof)
+    x, it, rel = ctx.gmres(rhs, tol=1e-10, maxit=60, restart=30)
+    assert rel <= 1e-10
 
 
 @pytest.mark.parametrize("name", CASES)
