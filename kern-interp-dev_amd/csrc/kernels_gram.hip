@@ -28,6 +28,28 @@ namespace ki {
 
 namespace {
 
+// sum over l = l0, l0 + 4, ... < l1 of a(l) * b(l), accumulated in that order
+// (the rounding of the plain loop), with U loads of each operand in flight
+// per round instead of one: a runtime-bound loop is otherwise compiled as
+// load, wait, multiply-add per term
+template <int U, typename FA, typename FB>
+__device__ __forceinline__ double strided_dot4(int l0, int l1, FA a, FB b) {
+  double part = 0.0;
+  for (int lb = l0; lb < l1; lb += 4 * U) {
+    double va[U], vb[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int l = min(lb + 4 * u, l1 - 1);  // in range; unused past l1
+      va[u] = a(l);
+      vb[u] = b(l);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (lb + 4 * u < l1) part += va[u] * vb[u];
+  }
+  return part;
+}
+
 constexpr int GT = 64, GK = 32;
 
 template <typename TI>
@@ -368,6 +390,7 @@ pchol_step(const CholDesc* __restrict__ desc, const double* __restrict__ g_all,
   if (live) {
     const double* Lj = L + j;
     const double* Lp = L + p;
+    // same terms and order as pchol_coop's strided_dot4 (bit-identical)
     for (int l = i0 + wid; l < i; l += 4)
       part += Lj[(int64_t)n * l] * Lp[(int64_t)n * l];
   }
@@ -511,8 +534,9 @@ pchol_coop(const CholDesc* __restrict__ desc, const int2* __restrict__ chunks,
     const bool live = j < n && dj >= 0.0 && j != p;
     double part = 0.0;
     if (live)
-      for (int l = i0 + wid; l < i; l += 4)
-        part += L[j + (int64_t)n * l] * Lp[l - i0];
+      part = strided_dot4<4>(
+          i0 + wid, i, [&](int l) { return L[j + (int64_t)n * l]; },
+          [&](int l) { return Lp[l - i0]; });
     red[wid][lane] = part;
     __syncthreads();
     if (j < n) {
