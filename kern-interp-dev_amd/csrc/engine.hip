@@ -105,7 +105,7 @@ struct Ctx {
   hipStream_t s = nullptr;
   int dom = 0, sol = 0, pde = 0, npts = 0, ndof = 0;
   std::vector<double> pts, nrm, w, curv;
-  DevBuf d_pts, d_nrm, d_w, d_curv, d_diag, d_opsum;
+  DevBuf d_pts, d_nrm, d_w, d_curv, d_diag, d_opsum, d_opart;
   DevGeom g;
   bool diag_ready = false;
   Tree tree;
@@ -1706,6 +1706,13 @@ int ki_set_geometry(ki_ctx* ctx, int dom, int pde, const double* pts,
     c.g.curv = c.d_curv.as<double>();
     c.d_opsum.ensure(sizeof(double) * ((size_t)(npts + 255) / 256 + 1));
     c.g.red = c.d_opsum.as<double>();
+    {
+      const size_t cap =
+          (size_t)ki::matvec_splits(npts, npts) * npts * c.sol;
+      c.d_opart.ensure(sizeof(double) * cap);
+      c.g.part = c.d_opart.as<double>();
+      c.g.part_cap = (int64_t)cap;
+    }
     c.g.diag = nullptr;
     c.diag_ready = false;
     c.tree_built = false;
@@ -2042,6 +2049,14 @@ int ki_apply_forward(ki_ctx* ctx, const double* dpts, int ndp,
     c.d_dout.ensure(sizeof(double) * (size_t)std::max(ndp, 1) * c.sol);
     c.d_tmpv.ensure(sizeof(double) * c.ndof);
     if (ndp <= 0) return;
+    {
+      const size_t cap = (size_t)ki::matvec_splits(ndp, c.npts) * ndp * c.sol;
+      if ((int64_t)cap > c.g.part_cap) {
+        c.d_opart.ensure(sizeof(double) * cap);
+        c.g.part = c.d_opart.as<double>();
+        c.g.part_cap = (int64_t)cap;
+      }
+    }
     KI_HIP(hipMemcpyAsync(c.d_dp.p, dpts, sizeof(double) * ndp * c.dom,
                           hipMemcpyHostToDevice, c.s));
     KI_HIP(hipMemcpyAsync(c.d_tmpv.p, mu, sizeof(double) * c.ndof,

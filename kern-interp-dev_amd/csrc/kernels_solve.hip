@@ -361,30 +361,86 @@ __global__ void __launch_bounds__(256) nmu_final_kernel(DevGeom g, int nblk) {
 // forward evaluation of Kernel::apply_forward).  Specialized on the domain
 // dimension DOM and unknowns per point SOL so the pair loop is straight-line
 // code; the self pair (r = 0) is masked by giving it zero weight.
+//
+// The sources are split into kMvSplit-tile ranges over blockIdx.y, so a
+// 65,536-target operator runs 2,048 workgroups (about eight waves per SIMD
+// instead of one), and each thread serves kMvT targets, so every source
+// staged in LDS is read once per kMvT pairs.  matvec_part writes one partial
+// sum per (split, target); matvec_finish adds them in split order (the same
+// bits on every call) and applies the self and rank-one terms.
+constexpr int kMvT = 2;
+
+// pair contributions of one staged source to one target
+template <int DOM, int SOL>
+__device__ __forceinline__ void mv_pair(const double (&tx)[DOM], bool is_self,
+                                        const double* sx, const double* sn,
+                                        double swq, const double* sm,
+                                        double (&acc)[SOL]) {
+  double r[DOM], rr = 0.0, dot = 0.0;
+#pragma unroll
+  for (int a = 0; a < DOM; a++) {
+    r[a] = tx[a] - sx[a];
+    rr += r[a] * r[a];
+    dot += r[a] * sn[a];
+  }
+  const double w = is_self ? 0.0 : swq;
+  rr = is_self ? 1.0 : rr;
+  if (SOL == 1) {
+    if (DOM == 2) {
+      acc[0] += w * dot / rr * sm[0];
+    } else {
+      const double ir = rsqrt_nr(rr);
+      acc[0] += w * dot * (ir * ir * ir) * sm[0];
+    }
+  } else {
+    double rm = 0.0;
+#pragma unroll
+    for (int a = 0; a < DOM; a++) rm += r[a] * sm[a];
+    double f;
+    if (DOM == 2) {
+      f = w * dot / (rr * rr);
+    } else {
+      const double ir = rsqrt_nr(rr);
+      const double ir2 = ir * ir;
+      f = w * dot * (ir2 * ir2 * ir);
+    }
+    // the rank-one term w_s (n_s . mu_s) n_t is added by matvec_finish
+    const double frm = f * rm;
+#pragma unroll
+    for (int a = 0; a < SOL; a++) acc[a] += frm * r[a];
+  }
+}
+
+// grid: (ceil(ntgt / (MT kMvT)), nsplit); split y covers source tiles
+// [y tps, min(ntiles, (y + 1) tps)); part[(y ntgt + i) SOL + a]
 template <int DOM, int SOL>
 __global__ void __launch_bounds__(MT)
-matvec_kernel(DevGeom g, const double* __restrict__ tpts,
-              const double* __restrict__ tnrm, int ntgt, int self,
-              const double* __restrict__ mu, double* __restrict__ y) {
+matvec_part(DevGeom g, const double* __restrict__ tpts, int ntgt, int self,
+            const double* __restrict__ mu, int tps,
+            double* __restrict__ part) {
   const double PI = 3.14159265358979323846;
   // kernel constant folded into the staged source weights
   const double kC = SOL == 1 ? (DOM == 2 ? -1.0 / (2 * PI) : -1.0 / (4 * PI))
                              : (DOM == 2 ? 1.0 / PI : -3.0 / (4.0 * PI));
-  const int dom = DOM, sol = SOL, n = g.npts;
-  const int i = blockIdx.x * MT + threadIdx.x;
+  const int n = g.npts;
+  const int base = blockIdx.x * MT * kMvT;
   __shared__ double sx[MT][DOM], sn[MT][DOM], sw[MT], sm[MT][SOL];
-  double tx[DOM], tn[DOM];
+  double tx[kMvT][DOM], acc[kMvT][SOL];
+  int me[kMvT];
 #pragma unroll
-  for (int a = 0; a < DOM; a++) {
-    tx[a] = i < ntgt ? tpts[(int64_t)i * DOM + a] : 0.0;
-    tn[a] = (i < ntgt && tnrm) ? tnrm[(int64_t)i * DOM + a] : 0.0;
+  for (int t = 0; t < kMvT; t++) {
+    const int i = base + t * MT + threadIdx.x;
+#pragma unroll
+    for (int a = 0; a < DOM; a++)
+      tx[t][a] = i < ntgt ? tpts[(int64_t)i * DOM + a] : 0.0;
+#pragma unroll
+    for (int a = 0; a < SOL; a++) acc[t][a] = 0.0;
+    me[t] = self && i < ntgt ? i : -1;
   }
-  double acc[SOL];
-#pragma unroll
-  for (int a = 0; a < SOL; a++) acc[a] = 0.0;
-  const int me = self && i < ntgt ? i : -1;
-  for (int j0 = 0; j0 < n; j0 += MT) {
-    const int j = j0 + threadIdx.x;
+  const int ntiles = (n + MT - 1) / MT;
+  const int tb = blockIdx.y * tps, te = min(ntiles, tb + tps);
+  for (int tile = tb; tile < te; tile++) {
+    const int j0 = tile * MT, j = j0 + threadIdx.x;
     __syncthreads();
     if (j < n) {
 #pragma unroll
@@ -399,49 +455,47 @@ matvec_kernel(DevGeom g, const double* __restrict__ tpts,
     }
     __syncthreads();
     const int cnt = min(MT, n - j0);
-#pragma unroll 4
+#pragma unroll 2
     for (int q = 0; q < cnt; q++) {
-      double r[DOM], rr = 0.0, dot = 0.0;
 #pragma unroll
-      for (int a = 0; a < DOM; a++) {
-        r[a] = tx[a] - sx[q][a];
-        rr += r[a] * r[a];
-        dot += r[a] * sn[q][a];
-      }
-      const bool is_self = j0 + q == me;
-      const double w = is_self ? 0.0 : sw[q];
-      rr = is_self ? 1.0 : rr;
-      if (SOL == 1) {
-        if (DOM == 2) {
-          acc[0] += w * dot / rr * sm[q][0];
-        } else {
-          const double ir = rsqrt_nr(rr);
-          acc[0] += w * dot * (ir * ir * ir) * sm[q][0];
-        }
-      } else {
-        double rm = 0.0;
-#pragma unroll
-        for (int a = 0; a < DOM; a++) rm += r[a] * sm[q][a];
-        double f;
-        if (DOM == 2) {
-          f = w * dot / (rr * rr);
-        } else {
-          const double ir = rsqrt_nr(rr);
-          const double ir2 = ir * ir;
-          f = w * dot * (ir2 * ir2 * ir);
-        }
-        // the rank-one term w_s (n_s . mu_s) n_t is added after the loop
-        const double frm = f * rm;
-#pragma unroll
-        for (int a = 0; a < SOL; a++) acc[a] += frm * r[a];
-      }
+      for (int t = 0; t < kMvT; t++)
+        mv_pair<DOM, SOL>(tx[t], j0 + q == me[t], sx[q], sn[q], sw[q], sm[q],
+                          acc[t]);
     }
   }
+#pragma unroll
+  for (int t = 0; t < kMvT; t++) {
+    const int i = base + t * MT + threadIdx.x;
+    if (i < ntgt)
+#pragma unroll
+      for (int a = 0; a < SOL; a++)
+        part[((int64_t)blockIdx.y * ntgt + i) * SOL + a] = acc[t][a];
+  }
+}
+
+template <int DOM, int SOL>
+__global__ void __launch_bounds__(256)
+matvec_finish(DevGeom g, const double* __restrict__ tnrm, int ntgt, int self,
+              const double* __restrict__ mu, int nsplit,
+              const double* __restrict__ part, double* __restrict__ y) {
+  const double PI = 3.14159265358979323846;
+  const int sol = SOL, n = g.npts;
+  const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= ntgt) return;
+  double acc[SOL];
+#pragma unroll
+  for (int a = 0; a < SOL; a++) acc[a] = 0.0;
+  for (int y0 = 0; y0 < nsplit; y0++)
+#pragma unroll
+    for (int a = 0; a < SOL; a++)
+      acc[a] += part[((int64_t)y0 * ntgt + i) * SOL + a];
   if (!self) {
     for (int a = 0; a < sol; a++) y[(int64_t)i * sol + a] = acc[a];
     return;
   }
+  double tn[DOM];
+#pragma unroll
+  for (int a = 0; a < DOM; a++) tn[a] = tnrm[(int64_t)i * DOM + a];
   if constexpr (SOL > 1) {
     // rank-one term: n_t times the sum over the other sources of
     // w_s (n_s . mu_s) (the whole sum, from nmu_*_kernel, minus this point)
@@ -662,10 +716,26 @@ template void launch_diag_blocked<float, double>(const DiagDesc*, int, int,
                                                  int64_t, double* const*,
                                                  float*, double*, hipStream_t);
 
+int matvec_splits(int ntgt, int nsrc) {
+  // about eight waves per SIMD on 256 CUs: 8,192 waves, four per workgroup
+  const int wg = (ntgt + MT * kMvT - 1) / (MT * kMvT);
+  const int ntiles = (nsrc + MT - 1) / MT;
+  const int want = 8192 / std::max(1, 4 * wg);
+  return std::max(1, std::min({want, ntiles, 32}));
+}
+
 static void matvec_any(const DevGeom& g, const double* tpts,
                        const double* tnrm, int ntgt, int self,
                        const double* mu, double* y, hipStream_t s) {
-  const dim3 grid((ntgt + MT - 1) / MT), blk(MT);
+  if (ntgt <= 0) return;
+  const int nsplit = matvec_splits(ntgt, g.npts);
+  if (g.part == nullptr ||
+      (int64_t)nsplit * ntgt * g.sol > g.part_cap)
+    fail(2, "operator partial-sum buffer too small");
+  const int ntiles = (g.npts + MT - 1) / MT;
+  const int tps = (ntiles + nsplit - 1) / nsplit;
+  const dim3 grid((ntgt + MT * kMvT - 1) / (MT * kMvT), nsplit), blk(MT);
+  const dim3 gfin((ntgt + 255) / 256);
   if (self && g.sol > 1) {  // the rank-one term's source sum first
     const int nblk = (g.npts + 255) / 256;
     if (g.dom == 2)
@@ -676,20 +746,24 @@ static void matvec_any(const DevGeom& g, const double* tpts,
                          mu);
     hipLaunchKernelGGL(nmu_final_kernel, dim3(1), dim3(256), 0, s, g, nblk);
   }
+#define KI_MV(D, S)                                                           \
+  do {                                                                        \
+    hipLaunchKernelGGL((matvec_part<D, S>), grid, blk, 0, s, g, tpts, ntgt,   \
+                       self, mu, tps, g.part);                                \
+    hipLaunchKernelGGL((matvec_finish<D, S>), gfin, dim3(256), 0, s, g, tnrm, \
+                       ntgt, self, mu, nsplit, (const double*)g.part, y);     \
+  } while (0)
   if (g.dom == 2 && g.sol == 1)
-    hipLaunchKernelGGL((matvec_kernel<2, 1>), grid, blk, 0, s, g, tpts, tnrm,
-                       ntgt, self, mu, y);
+    KI_MV(2, 1);
   else if (g.dom == 2 && g.sol == 2)
-    hipLaunchKernelGGL((matvec_kernel<2, 2>), grid, blk, 0, s, g, tpts, tnrm,
-                       ntgt, self, mu, y);
+    KI_MV(2, 2);
   else if (g.dom == 3 && g.sol == 1)
-    hipLaunchKernelGGL((matvec_kernel<3, 1>), grid, blk, 0, s, g, tpts, tnrm,
-                       ntgt, self, mu, y);
+    KI_MV(3, 1);
   else if (g.dom == 3 && g.sol == 3)
-    hipLaunchKernelGGL((matvec_kernel<3, 3>), grid, blk, 0, s, g, tpts, tnrm,
-                       ntgt, self, mu, y);
+    KI_MV(3, 3);
   else
     fail(2, "unsupported dimension / PDE for the matvec");
+#undef KI_MV
   KI_HIP(hipGetLastError());
 }
 

@@ -97,6 +97,9 @@ struct DevGeom {
   double* diag = nullptr;   // 3D: npts (Laplace) or npts*9 (Stokes)
   double* red = nullptr;    // scratch of the operator's source sums:
                             // (npts + 255) / 256 + 1 values
+  double* part = nullptr;   // the operator's per-split partial sums:
+  int64_t part_cap = 0;     // part_cap doubles, matvec_splits(ntgt, npts)
+                            // x ntgt x sol needed per call
 };
 
 // Kernel-entry evaluation mode for one row segment of a generated block.
@@ -379,6 +382,8 @@ void mlp_adam(float* params, const float* grad, float* m, float* v, float lr,
 void mlp_infer(const float* params, const float* X, int n, float* out,
                int bf16, hipStream_t s);
 
+// source splits of the operator over ntgt targets and nsrc sources
+int matvec_splits(int ntgt, int nsrc);
 void launch_matvec(const DevGeom& g, const double* mu, double* y,
                    hipStream_t s);
 void launch_forward(const DevGeom& g, const double* tpts, int ntgt,

@@ -146,6 +146,31 @@ def test_matvec_matches_dense(name):
     assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("npts,pde", [(32768, "laplace"), (16384, "stokes")])
+def test_matvec_rows_match_kernel_blocks_at_scale(npts, pde):
+    """The operator at sizes where every source split of the residual kernel
+    spans several 256-source tiles (the small cases above have one tile per
+    split): K mu on 256 random rows against the rows of ki_kernel_block
+    (entries pinned to the oracle at 1e-13) times mu.  A dropped or doubled
+    tile moves a row by far more than the 1e-11 bar, which leaves room for
+    the different summation order over ~50,000 terms."""
+    if pde == "laplace":
+        b, p = G.fibonacci_sphere(npts), ki.KI_LAPLACE
+    else:
+        b, p = G.ellipsoid(npts), ki.KI_STOKES
+    print("[case] matvec rows npts=%d %s" % (npts, pde), flush=True)
+    ctx = ki.Context()
+    ctx.set_geometry(b, p)
+    rng = np.random.default_rng(11)
+    mu = rng.normal(size=ctx.ndof)
+    y = ctx.matvec(mu)
+    rows = np.sort(rng.choice(ctx.ndof, 256, replace=False))
+    ref = ctx.kernel_block(rows, np.arange(ctx.ndof)) @ mu
+    assert np.abs(y[rows] - ref).max() <= 1e-11 * np.abs(ref).max()
+    # a second call gives the same bits (fixed split order)
+    assert np.array_equal(ctx.matvec(mu), y)
+
+
 def _oracle_factor(name, id_sketch=0):
     import dataclasses
     import ki_oracle as O
