@@ -37,8 +37,10 @@ PEAK_FP32 = 157.3e12       # FLOP/s dense FP32 (vector = matrix on MI355X)
 PEAK_FP64 = 78.6e12
 
 
-# ki_params::id_sketch of the FP32 form (0: the library's default)
+# ki_params::id_sketch and ki_params::top_schur of the FP32 form (0: the
+# library's defaults)
 ID_SKETCH = 0
+TOP_SCHUR = 0
 
 
 def params_for(ki, pb, precision, predictor=0):
@@ -46,7 +48,8 @@ def params_for(ki, pb, precision, predictor=0):
                              proxy_ratio=pb.proxy_ratio,
                              proxy_kind=1 if pb.boundary.dom == 3 else 0,
                              precision=precision, use_predictor=predictor,
-                             id_sketch=ID_SKETCH if precision == 32 else 0)
+                             id_sketch=ID_SKETCH if precision == 32 else 0,
+                             top_schur=TOP_SCHUR if precision == 32 else 0)
 
 
 # Which peak bounds each event-timed kernel: the Gram product runs on the
@@ -189,14 +192,18 @@ def main():
     ap.add_argument("--id-sketch", type=int, default=0,
                     help="row sketch of the FP32 form's Gram ID: rows per "
                          "column (0: library default, -1: off)")
+    ap.add_argument("--top-schur", type=int, default=0, choices=(0, 32, 64),
+                    help="arithmetic of the FP32 form's top-level LU Schur "
+                         "updates (0: library default, FP32 matrix cores)")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-predictor", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 1 GiB read + 1 GiB write elementwise "
                     "kernel (scale for the HBM byte counters) and exit")
     args = ap.parse_args()
-    global ID_SKETCH
+    global ID_SKETCH, TOP_SCHUR
     ID_SKETCH = args.id_sketch
+    TOP_SCHUR = args.top_schur
     if args.calibrate_copy:
         calibrate_copy()
         return

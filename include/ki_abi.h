@@ -79,7 +79,12 @@ typedef struct ki_params {
                                at least twice that many ID rows.  0: auto
                                (3 in FP32 mode, off in FP64 mode); -1: off;
                                k >= 1: k * n rows in either mode            */
-  int reserved[1];
+  int top_schur;             /* arithmetic of the top-level LU's trailing
+                               Schur updates (its panels, pivots and
+                               interchanges are always FP64).  0: auto (32 in
+                               FP32 mode, 64 in FP64 mode); 32: FP32 matrix-
+                               core products subtracted into the FP64 matrix;
+                               64: FP64                                       */
 } ki_params;
 
 void ki_params_default(ki_params* p);

@@ -59,7 +59,7 @@ class KiParams(C.Structure):
         ("lu_block", C.c_int),
         ("solve_wave", C.c_int),
         ("id_sketch", C.c_int),
-        ("reserved", C.c_int * 1),
+        ("top_schur", C.c_int),
     ]
 
 

@@ -1229,9 +1229,16 @@ void skeletonize_t(Ctx& c) {
     st.flops_top = 2.0 / 3.0 * (double)nt * nt * nt;
     {
       Timer tl(s);
+      // the trailing Schur updates of the top LU on FP32 matrix cores
+      // (ki_params::top_schur; auto: in the FP32 form only)
+      struct Schur32 {
+        explicit Schur32(bool on) { set_lu_schur32(on); }
+        ~Schur32() { set_lu_schur32(false); }
+      } sc(c.prm.top_schur == 32 ||
+           (c.prm.top_schur == 0 && sizeof(T) == 4));
       batched_lu<double>(md, c.top_desc.as<MatDesc>(), c.top.as<double>(),
                          c.top_piv.as<int>(), s);
-      c.clock("top-level LU (FP64)", tl.stop(), st.flops_top,
+      c.clock("top-level LU (FP64 panels)", tl.stop(), st.flops_top,
               16.0 * (double)nt * nt, 0);
     }
     st.t_top = ttop.stop();
@@ -1843,6 +1850,9 @@ int ki_skeletonize(ki_ctx* ctx, const ki_params* params) {
              "LU row thresholds must be >= 0");
     KI_CHECK(c.prm.id_sketch >= -1 && c.prm.id_sketch <= 64, KI_E_ARG,
              "id_sketch must be -1, 0 or a row multiple 1..64");
+    KI_CHECK(c.prm.top_schur == 0 || c.prm.top_schur == 32 ||
+                 c.prm.top_schur == 64,
+             KI_E_ARG, "top_schur must be 0, 32 or 64");
     c.factored = false;
     c.prec = c.prm.precision;
     struct Thresholds {  // this call's LU thresholds, defaults afterwards

@@ -108,8 +108,24 @@ gemm_kernel(const GemmDesc* __restrict__ desc, const T* __restrict__ a_all,
     }
     __syncthreads();
   }
-  // accumulator (a, b) element v: C row = lane % 16, C column = row(lane, v)
+  // accumulator (a, b) element v: C row = lane % 16, C column = row(lane, v).
+  // The 16 C values are loaded together first (one round trip), then
+  // updated and stored.
   const T alpha = (T)d.alpha, beta = (T)d.beta;
+  T cv[2][2][4];
+#pragma unroll
+  for (int a = 0; a < 2; a++) {
+    const int gi = i0 + wr + 16 * a + (lane & 15);
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int gj = j0 + wc + 16 * b + Mfma<T>::row(lane, v);
+        cv[a][b][v] = (beta != (T)0 && gi < d.m && gj < d.n)
+                          ? C[gi + (int64_t)d.ldc * gj]
+                          : (T)0;
+      }
+  }
 #pragma unroll
   for (int a = 0; a < 2; a++) {
     const int gi = i0 + wr + 16 * a + (lane & 15);
@@ -119,9 +135,9 @@ gemm_kernel(const GemmDesc* __restrict__ desc, const T* __restrict__ a_all,
       for (int v = 0; v < 4; v++) {
         const int gj = j0 + wc + 16 * b + Mfma<T>::row(lane, v);
         if (gi >= d.m || gj >= d.n) continue;
-        T* cp = C + gi + (int64_t)d.ldc * gj;
         const T x = acc[a][b][v];
-        *cp = beta == (T)0 ? alpha * x : alpha * x + beta * *cp;
+        C[gi + (int64_t)d.ldc * gj] =
+            beta == (T)0 ? alpha * x : alpha * x + beta * cv[a][b][v];
       }
     }
   }
@@ -177,20 +193,24 @@ struct SliceReader {
       for (int q = 0; q < NR; q++) r[q] = q < nk ? pk[q * step] : (T)0;
     }
   }
-  __device__ void store(T (*S)[HP]) const {
+  template <typename TS>
+  __device__ void store(TS (*S)[HP]) const {
 #pragma unroll
     for (int q = 0; q < NR; q++) {
       if (KMAJOR)
-        S[ti + q * (256 / HK)][tk] = r[q];
+        S[ti + q * (256 / HK)][tk] = (TS)r[q];
       else
-        S[ti][tk + 2 * q] = r[q];
+        S[ti][tk + 2 * q] = (TS)r[q];
     }
   }
 };
 
 // SYM: C is symmetric (m = n) and so is the product (B = A, TB = !TA): only
 // the upper tiles run, each writing its block and the mirrored one.
-template <typename T, int TA, int TB, bool SYM = false>
+// TM: the type the products run in.  Operands are read in their storage type
+// T and rounded to TM when staged in LDS; the TM accumulators are added into
+// C in T (FP64 matrix, FP32 matrix-core Schur product: TM = float, T = double).
+template <typename T, int TA, int TB, bool SYM = false, typename TM = T>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 gemm128_kernel(const GemmDesc* __restrict__ desc, const T* __restrict__ a_all,
                const T* __restrict__ b_all, T* __restrict__ c_all) {
@@ -215,13 +235,13 @@ gemm128_kernel(const GemmDesc* __restrict__ desc, const T* __restrict__ a_all,
   const T* B = b_all + d.b_off;
   T* C = c_all + d.c_off;
   const int64_t lda = d.lda, ldb = d.ldb;
-  __shared__ T As[H][HP];  // As[i][k] = op(A)(i0 + i, k0 + k)
-  __shared__ T Bs[H][HP];  // Bs[j][k] = op(B)(k0 + k, j0 + j)
+  __shared__ TM As[H][HP];  // As[i][k] = op(A)(i0 + i, k0 + k)
+  __shared__ TM Bs[H][HP];  // Bs[j][k] = op(B)(k0 + k, j0 + j)
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int wr = (wid >> 1) * 64, wc = (wid & 1) * 64;
   SliceReader<T, TA == 1> ra(A, lda, i0, d.m, tid);
   SliceReader<T, TB == 0> rb(B, ldb, j0, d.n, tid);
-  typedef typename Mfma<T>::acc_t acc_t;
+  typedef typename Mfma<TM>::acc_t acc_t;
   acc_t acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; a++)
@@ -241,7 +261,7 @@ gemm128_kernel(const GemmDesc* __restrict__ desc, const T* __restrict__ a_all,
 #pragma unroll
     for (int ks = 0; ks < HK; ks += 4) {
       const int kr = ks + (lane >> 4);
-      T av[4], bv[4];
+      TM av[4], bv[4];
 #pragma unroll
       for (int a = 0; a < 4; a++) av[a] = As[wr + 16 * a + (lane & 15)][kr];
 #pragma unroll
@@ -250,23 +270,36 @@ gemm128_kernel(const GemmDesc* __restrict__ desc, const T* __restrict__ a_all,
       for (int a = 0; a < 4; a++)
 #pragma unroll
         for (int b = 0; b < 4; b++)
-          acc[a][b] = Mfma<T>::mac(bv[b], av[a], acc[a][b]);
+          acc[a][b] = Mfma<TM>::mac(bv[b], av[a], acc[a][b]);
     }
   }
+  // Per row group a: its 16 C values are loaded together first (one round
+  // trip), then updated and stored.  Interleaved, every load would wait
+  // behind the previous store, which the compiler cannot prove is to
+  // another address.
   const T alpha = (T)d.alpha, beta = (T)d.beta;
 #pragma unroll
   for (int a = 0; a < 4; a++) {
     const int gi = i0 + wr + 16 * a + (lane & 15);
+    T cv[4][4];
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int gj = j0 + wc + 16 * b + Mfma<TM>::row(lane, v);
+        cv[b][v] = (beta != (T)0 && gi < d.m && gj < d.n)
+                       ? C[gi + (int64_t)d.ldc * gj]
+                       : (T)0;
+      }
 #pragma unroll
     for (int b = 0; b < 4; b++) {
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int gj = j0 + wc + 16 * b + Mfma<T>::row(lane, v);
+        const int gj = j0 + wc + 16 * b + Mfma<TM>::row(lane, v);
         if (gi >= d.m || gj >= d.n) continue;
-        T* cp = C + gi + (int64_t)d.ldc * gj;
-        const T x = acc[a][b][v];
-        const T nv = beta == (T)0 ? alpha * x : alpha * x + beta * *cp;
-        *cp = nv;
+        const T x = (T)acc[a][b][v];
+        const T nv = beta == (T)0 ? alpha * x : alpha * x + beta * cv[b][v];
+        C[gi + (int64_t)d.ldc * gj] = nv;
         if (SYM && i0 != j0) C[gj + (int64_t)d.ldc * gi] = nv;
       }
     }
@@ -1168,6 +1201,8 @@ template bool library_gemm<double>(const GemmDesc&, const double*,
 namespace {
 thread_local int g_lu_wide_rows = 0, g_lu_coop_rows = 0;
 thread_local bool g_lu_no_block = false;
+// FP32 matrix-core trailing updates in the block-panel form of an FP64 LU
+thread_local bool g_lu_schur32 = false;
 // Set when the runtime refused a cooperative launch (for instance while
 // another process on the GPU holds the cooperative-launch resources): the
 // LUs of this process then use the per-column / one-workgroup panels.
@@ -1182,6 +1217,8 @@ void set_lu_thresholds(int wide_rows, int coop_rows, bool block) {
   g_lu_coop_rows = coop_rows;
   g_lu_no_block = !block;
 }
+
+void set_lu_schur32(bool on) { g_lu_schur32 = on; }
 
 // Scratch and launches of the cooperative panel kernel for a batch of
 // matrices that advance panel by panel together (PW-column panels, PRW rows
@@ -1270,6 +1307,25 @@ static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
   if (!g_lu_no_block && bp.init(std::vector<MatDesc>{x}, s)) {
     GemmDesc* d_g = nullptr;
     KI_HIP(hipMallocAsync((void**)&d_g, sizeof(GemmDesc), s));
+    // FP32 matrix-core form of each outer block's K = 2 BW trailing update
+    // (FP64 matrix): descriptors of every outer block, uploaded once
+    const bool schur32 = sizeof(T) == 8 && g_lu_schur32;
+    const int nouter = (mn + 2 * BW - 1) / (2 * BW);
+    GemmDesc* d_t = nullptr;
+    std::vector<GemmDesc> tq;
+    if (schur32) {
+      for (int J = 0; J < mn; J += 2 * BW) {
+        const int OE = std::min(J + 2 * BW, mn);
+        tq.push_back(GemmDesc{m - OE, n - OE, OE - J,
+                              x.off + OE + (int64_t)ld * J,
+                              x.off + J + (int64_t)ld * OE,
+                              x.off + OE + (int64_t)ld * OE,
+                              ld, ld, ld, 0, 0, -1.0, 1.0});
+      }
+      KI_HIP(hipMallocAsync((void**)&d_t, sizeof(GemmDesc) * nouter, s));
+      KI_HIP(hipMemcpyAsync(d_t, tq.data(), sizeof(GemmDesc) * nouter,
+                            hipMemcpyHostToDevice, s));
+    }
     auto cols = [](int c) { return dim3((std::max(c, 1) + 255) / 256); };
     auto swaps = [&](int r0, int r1, int c0, int c1) {
       if (c1 > c0)
@@ -1330,11 +1386,23 @@ static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
         update(B0, PB, OE, n - OE, J0, PA);
         trsm(B0, PB, OE, n);
       }
-      update(OE, m - OE, OE, n - OE, J0, OE - J0);
+      bool done = false;
+      if constexpr (sizeof(T) == 8) {
+        if (schur32) {
+          const GemmDesc& q = tq[J0 / (2 * BW)];
+          if (q.m > 0 && q.n > 0)
+            hipLaunchKernelGGL((gemm128_kernel<double, 0, 0, false, float>),
+                               dim3(((q.m + H - 1) / H) * ((q.n + H - 1) / H)),
+                               dim3(256), 0, s, d_t + J0 / (2 * BW), a, a, a);
+          done = true;
+        }
+      }
+      if (!done) update(OE, m - OE, OE, n - OE, J0, OE - J0);
     }
     KI_HIP(hipGetLastError());
     KI_HIP(hipStreamSynchronize(s));
     KI_HIP(hipFreeAsync(d_g, s));
+    if (d_t != nullptr) KI_HIP(hipFreeAsync(d_t, s));
     bp.finish(s);
   }
   if (J0 < mn) lu_two_level<T>(x, a, piv_all, J0, s);

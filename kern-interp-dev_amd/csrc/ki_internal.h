@@ -296,6 +296,10 @@ void batched_lu(const std::vector<MatDesc>& h, MatDesc* d_desc, T* a,
 // blocked form; batches taller than coop_rows factor each panel over many
 // workgroups.
 void set_lu_thresholds(int wide_rows, int coop_rows, bool block);
+// On: the block-panel form of a single FP64 LU takes each outer block's
+// trailing (Schur) update as an FP32 matrix-core product subtracted in FP64;
+// the panels, their pivots and interchanges stay FP64.  Calling thread only.
+void set_lu_schur32(bool on);
 
 // X = U^{-1} B for upper-triangular U (no interchanges), blocked.
 template <typename T>

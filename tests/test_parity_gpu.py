@@ -403,6 +403,35 @@ def test_multi_workgroup_lu_panels_match_one_workgroup_panels(knob,
     assert np.linalg.norm(x2 - x1) <= tol * np.linalg.norm(x1)
 
 
+@pytest.mark.parametrize("precision", [64, 32])
+def test_fp32_top_schur_updates(precision):
+    """The top-level LU's trailing Schur updates as FP32 matrix-core products
+    subtracted into the FP64 matrix (top_schur = 32; the block-panel LU is
+    forced onto this small case with lu_wide_rows) against FP64 updates
+    (top_schur = 64).  The direct solves agree to FP32 level (a wrong tile,
+    a transposed operand or a dropped outer block moves them by O(1)), and
+    FP64 GMRES still reaches 1e-10 and the dense solution."""
+    b, pde, kw, leaf, prm, ctx, okern, otree = setup("stokes3d_ellipsoid")
+    rng = np.random.default_rng(9)
+    rhs = rng.normal(size=ctx.ndof)
+    ctx.skeletonize(ki.default_params(precision=precision, lu_wide_rows=256,
+                                      top_schur=64, **kw))
+    x64 = ctx.solve(rhs)
+    ctx.skeletonize(ki.default_params(precision=precision, lu_wide_rows=256,
+                                      top_schur=32, **kw))
+    nt = ctx.stats()["top_size"]
+    print("[top_schur] fp%d top_size=%d" % (precision, nt))
+    assert nt > 512  # two outer blocks or more: the trailing update runs
+    x32 = ctx.solve(rhs)
+    assert np.linalg.norm(x32 - x64) <= 1e-4 * np.linalg.norm(x64)
+    x, it, rel = ctx.gmres(rhs, tol=1e-10, maxit=60, restart=30)
+    K = okern.dense()
+    true_rel = np.linalg.norm(K @ x - rhs) / np.linalg.norm(rhs)
+    assert rel <= 1e-10 and true_rel <= 2e-10
+    xd = np.linalg.solve(K, rhs)
+    assert np.linalg.norm(x - xd) <= 1e-8 * np.linalg.norm(xd)
+
+
 @pytest.mark.parametrize("n", [300, 4500])
 def test_dense_lu_matches_lapack(n):
     """Device LU (one-workgroup panels below 4096 rows, two-level multi-
