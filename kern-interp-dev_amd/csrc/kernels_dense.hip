@@ -209,11 +209,13 @@ struct SliceReader {
 // the upper tiles run, each writing its block and the mirrored one.
 // TM: the type the products run in.  Operands are read in their storage type
 // T and rounded to TM when staged in LDS; the TM accumulators are added into
-// C in T (FP64 matrix, FP32 matrix-core Schur product: TM = float, T = double).
-template <typename T, int TA, int TB, bool SYM = false, typename TM = T>
+// C of type TC (FP32 matrix-core Schur product into an FP64 matrix: T = TM =
+// float, TC = double).
+template <typename T, int TA, int TB, bool SYM = false, typename TM = T,
+          typename TC = T>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 gemm128_kernel(const GemmDesc* __restrict__ desc, const T* __restrict__ a_all,
-               const T* __restrict__ b_all, T* __restrict__ c_all) {
+               const T* __restrict__ b_all, TC* __restrict__ c_all) {
   const GemmDesc d = desc[blockIdx.y];
   const int tiles_m = (d.m + H - 1) / H;
   const int tiles_n = (d.n + H - 1) / H;
@@ -233,7 +235,7 @@ gemm128_kernel(const GemmDesc* __restrict__ desc, const T* __restrict__ a_all,
   }
   const T* A = a_all + d.a_off;
   const T* B = b_all + d.b_off;
-  T* C = c_all + d.c_off;
+  TC* C = c_all + d.c_off;
   const int64_t lda = d.lda, ldb = d.ldb;
   __shared__ TM As[H][HP];  // As[i][k] = op(A)(i0 + i, k0 + k)
   __shared__ TM Bs[H][HP];  // Bs[j][k] = op(B)(k0 + k, j0 + j)
@@ -277,19 +279,19 @@ gemm128_kernel(const GemmDesc* __restrict__ desc, const T* __restrict__ a_all,
   // trip), then updated and stored.  Interleaved, every load would wait
   // behind the previous store, which the compiler cannot prove is to
   // another address.
-  const T alpha = (T)d.alpha, beta = (T)d.beta;
+  const TC alpha = (TC)d.alpha, beta = (TC)d.beta;
 #pragma unroll
   for (int a = 0; a < 4; a++) {
     const int gi = i0 + wr + 16 * a + (lane & 15);
-    T cv[4][4];
+    TC cv[4][4];
 #pragma unroll
     for (int b = 0; b < 4; b++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         const int gj = j0 + wc + 16 * b + Mfma<TM>::row(lane, v);
-        cv[b][v] = (beta != (T)0 && gi < d.m && gj < d.n)
+        cv[b][v] = (beta != (TC)0 && gi < d.m && gj < d.n)
                        ? C[gi + (int64_t)d.ldc * gj]
-                       : (T)0;
+                       : (TC)0;
       }
 #pragma unroll
     for (int b = 0; b < 4; b++) {
@@ -297,8 +299,8 @@ gemm128_kernel(const GemmDesc* __restrict__ desc, const T* __restrict__ a_all,
       for (int v = 0; v < 4; v++) {
         const int gj = j0 + wc + 16 * b + Mfma<TM>::row(lane, v);
         if (gi >= d.m || gj >= d.n) continue;
-        const T x = (T)acc[a][b][v];
-        const T nv = beta == (T)0 ? alpha * x : alpha * x + beta * cv[b][v];
+        const TC x = (TC)acc[a][b][v];
+        const TC nv = beta == (TC)0 ? alpha * x : alpha * x + beta * cv[b][v];
         C[gi + (int64_t)d.ldc * gj] = nv;
         if (SYM && i0 != j0) C[gj + (int64_t)d.ldc * gi] = nv;
       }
@@ -1008,6 +1010,17 @@ __global__ void zero_kernel(T* p, int64_t n) {
     p[e] = 0;
 }
 
+// dst (rows x cols, ld ldd) = src (rows x cols, ld lds) rounded to float;
+// grid (row blocks of 256, up to 65535 column strides)
+__global__ void __launch_bounds__(256)
+narrow_block(const double* __restrict__ src, int64_t lds, int rows, int cols,
+             float* __restrict__ dst, int64_t ldd) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= rows) return;
+  for (int c = blockIdx.y; c < cols; c += gridDim.y)
+    dst[i + ldd * c] = (float)src[i + lds * c];
+}
+
 template <typename TS, typename TD>
 __global__ void __launch_bounds__(256)
 scatter_kernel(const ScatterDesc* __restrict__ desc, const TS* __restrict__ src_all,
@@ -1309,22 +1322,26 @@ static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
     KI_HIP(hipMallocAsync((void**)&d_g, sizeof(GemmDesc), s));
     // FP32 matrix-core form of each outer block's K = 2 BW trailing update
     // (FP64 matrix): descriptors of every outer block, uploaded once
+    // L21 and U12 of the outer block are first rounded into FP32 panel
+    // buffers (m x 2 BW and 2 BW x n), so the product reads half the bytes
     const bool schur32 = sizeof(T) == 8 && g_lu_schur32;
     const int nouter = (mn + 2 * BW - 1) / (2 * BW);
     GemmDesc* d_t = nullptr;
+    float *l32 = nullptr, *u32 = nullptr;
     std::vector<GemmDesc> tq;
     if (schur32) {
       for (int J = 0; J < mn; J += 2 * BW) {
         const int OE = std::min(J + 2 * BW, mn);
-        tq.push_back(GemmDesc{m - OE, n - OE, OE - J,
-                              x.off + OE + (int64_t)ld * J,
-                              x.off + J + (int64_t)ld * OE,
-                              x.off + OE + (int64_t)ld * OE,
-                              ld, ld, ld, 0, 0, -1.0, 1.0});
+        tq.push_back(GemmDesc{m - OE, n - OE, OE - J, 0, 0,
+                              x.off + OE + (int64_t)ld * OE, m, 2 * BW, ld,
+                              0, 0, -1.0, 1.0});
       }
       KI_HIP(hipMallocAsync((void**)&d_t, sizeof(GemmDesc) * nouter, s));
       KI_HIP(hipMemcpyAsync(d_t, tq.data(), sizeof(GemmDesc) * nouter,
                             hipMemcpyHostToDevice, s));
+      KI_HIP(hipMallocAsync((void**)&l32, sizeof(float) * m * 2 * BW, s));
+      KI_HIP(hipMallocAsync((void**)&u32,
+                            sizeof(float) * 2 * BW * (size_t)n, s));
     }
     auto cols = [](int c) { return dim3((std::max(c, 1) + 255) / 256); };
     auto swaps = [&](int r0, int r1, int c0, int c1) {
@@ -1390,10 +1407,21 @@ static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
       if constexpr (sizeof(T) == 8) {
         if (schur32) {
           const GemmDesc& q = tq[J0 / (2 * BW)];
-          if (q.m > 0 && q.n > 0)
-            hipLaunchKernelGGL((gemm128_kernel<double, 0, 0, false, float>),
-                               dim3(((q.m + H - 1) / H) * ((q.n + H - 1) / H)),
-                               dim3(256), 0, s, d_t + J0 / (2 * BW), a, a, a);
+          if (q.m > 0 && q.n > 0) {
+            // L21 = A[OE:m, J0:OE] and U12 = A[J0:OE, OE:n] in FP32
+            hipLaunchKernelGGL(narrow_block, dim3((q.m + 255) / 256, q.k),
+                               dim3(256), 0, s, A + OE + (int64_t)ld * J0,
+                               (int64_t)ld, q.m, q.k, l32, (int64_t)m);
+            hipLaunchKernelGGL(narrow_block,
+                               dim3((q.k + 255) / 256, std::min(q.n, 65535)),
+                               dim3(256), 0, s, A + J0 + (int64_t)ld * OE,
+                               (int64_t)ld, q.k, q.n, u32, (int64_t)(2 * BW));
+            hipLaunchKernelGGL(
+                (gemm128_kernel<float, 0, 0, false, float, double>),
+                dim3(((q.m + H - 1) / H) * ((q.n + H - 1) / H)), dim3(256),
+                0, s, d_t + J0 / (2 * BW), (const float*)l32,
+                (const float*)u32, a);
+          }
           done = true;
         }
       }
@@ -1403,6 +1431,8 @@ static void lu_wide(const MatDesc& x, T* a, int* piv_all, hipStream_t s) {
     KI_HIP(hipStreamSynchronize(s));
     KI_HIP(hipFreeAsync(d_g, s));
     if (d_t != nullptr) KI_HIP(hipFreeAsync(d_t, s));
+    if (l32 != nullptr) KI_HIP(hipFreeAsync(l32, s));
+    if (u32 != nullptr) KI_HIP(hipFreeAsync(u32, s));
     bp.finish(s);
   }
   if (J0 < mn) lu_two_level<T>(x, a, piv_all, J0, s);
