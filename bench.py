@@ -129,6 +129,116 @@ def one_step(ctx, prm, tol, maxit, restart):
     return t1 - t0, t2 - t1, it, rel
 
 
+def family_batch(ki, G, n_test, n_train=64, tol=1e-10):
+    """The smooth-curve family of the learned-rank workload: n_train
+    held-out seeded curves r(t) = 1 + sum_k a_k cos(k t + phi_k) train the
+    rank predictor on the GPU, then n_test other curves (N ~ U{2048..32768}
+    GL nodes, a multiple of 16) are each factored and solved to `tol` three
+    ways: FP32 with learned ranks, FP32 without them, and plain FP64.  The
+    curves run one after another through one context per form (each
+    factorization batches its own tree levels; the curves are not fused
+    into one launch), the forms taking turns at going first.  Returns the
+    summary dict."""
+    rng = np.random.default_rng(4)
+
+    def n_of():
+        return int(rng.integers(128, 2049)) * 16
+
+    pb0 = G.config(3)
+    kw = dict(id_tol=pb0.id_tol, proxy_points=pb0.proxy_points,
+              proxy_ratio=pb0.proxy_ratio)
+    leaf = pb0.max_leaf_dofs
+    t0 = time.perf_counter()
+    h = ki.Context(0)
+    Xs, ys = [], []
+    for q in range(n_train):
+        b = G.random_smooth_curve(100000 + q, n_of())
+        h.set_geometry(b, ki.KI_LAPLACE)
+        h.build_tree(leaf)
+        h.skeletonize(ki.default_params(precision=32, **kw))
+        f, y, _ = h.box_features()
+        Xs.append(f)
+        ys.append(y)
+    X, y = np.concatenate(Xs), np.concatenate(ys)
+    h.mlp_init(7)
+    loss = h.mlp_train(X, y, epochs=300, batch=128, lr=2e-3, seed=7)
+    params = h.mlp_params()
+    h.close()
+    t_train = time.perf_counter() - t0
+    print("[family] predictor trained on %d boxes of %d curves in %.1f s, "
+          "loss %.2e" % (len(y), n_train, t_train, loss), file=sys.stderr,
+          flush=True)
+    forms = {"fp32_learned": (32, 1), "fp32": (32, 0), "fp64": (64, 0)}
+    ctxs = {k: ki.Context(0) for k in forms}
+    acc = {k: {"factor_s": 0.0, "solve_s": 0.0, "iters": 0} for k in forms}
+    errs, nb, fb, sk = [], 0, 0, 0
+    worst_rel, worst_diff, ndof = 0.0, 0.0, 0
+    for q in range(n_test):
+        b = G.random_smooth_curve(q, n_of())
+        pb = G.make_problem(b, "laplace", pb0.id_tol, leaf, pb0.proxy_points,
+                            pb0.proxy_ratio, 5, 3)
+        ndof += pb.ndof
+        sol = {}
+        # the forms take turns at running first on a new curve, so that
+        # one-time costs of a new size (buffer growth, GEMM tile tuning of
+        # a new product class) fall on each form equally
+        order = list(forms)[q % 3:] + list(forms)[:q % 3]
+        for k in order:
+            prec, pred = forms[k]
+            c = ctxs[k]
+            c.set_geometry(b, ki.KI_LAPLACE)
+            c.build_tree(leaf)
+            c.set_rhs(pb.rhs)
+            if pred:
+                c.mlp_set_params(params)
+            prm = ki.default_params(precision=prec, use_predictor=pred, **kw)
+            tf, ts, it, rel = one_step(c, prm, tol, 60, 30)
+            acc[k]["factor_s"] += tf
+            acc[k]["solve_s"] += ts
+            acc[k]["iters"] += it
+            worst_rel = max(worst_rel, rel)
+            sol[k] = c.solution()
+            if pred:
+                _, lab, pr = c.box_features()
+                ok = (lab >= 0) & (pr >= 0)
+                errs.append(np.abs(pr[ok] - lab[ok]))
+                st = c.stats()
+                nb += st["boxes_predicted"]
+                fb += st["boxes_fallback"]
+                sk += st["boxes_skipped"]
+        ref = sol["fp64"]
+        for k in ("fp32_learned", "fp32"):
+            worst_diff = max(worst_diff, float(np.linalg.norm(sol[k] - ref) /
+                                               np.linalg.norm(ref)))
+        if q % 16 == 15 or q == n_test - 1:
+            print("[family] %d / %d curves, %d DOFs so far" %
+                  (q + 1, n_test, ndof), file=sys.stderr, flush=True)
+    for c in ctxs.values():
+        c.close()
+    out = {}
+    for k in forms:
+        t = acc[k]["factor_s"] + acc[k]["solve_s"]
+        out[k] = {"dof_per_s": ndof / t, "factor_s": acc[k]["factor_s"],
+                  "solve_s": acc[k]["solve_s"],
+                  "mean_gmres_iters": acc[k]["iters"] / n_test}
+    e = np.concatenate(errs) if errs else np.zeros(0)
+    return {
+        "metric": "c4 smooth-curve family: DOF/s over all curves, FP32+learned "
+                  "vs FP32 and plain FP64, to relative residual %g" % tol,
+        "curves": n_test, "dofs": ndof, "train_curves": n_train,
+        "train_boxes": int(len(y)), "train_loss": float(loss),
+        "train_s": t_train, "forms": out,
+        "speedup_learned_vs_fp64":
+            out["fp32_learned"]["dof_per_s"] / out["fp64"]["dof_per_s"],
+        "speedup_learned_vs_fp32":
+            out["fp32_learned"]["dof_per_s"] / out["fp32"]["dof_per_s"],
+        "rank_fraction_mae": float(e.mean()) if e.size else None,
+        "boxes_predicted": nb, "boxes_fallback": fb,
+        "fallback_rate": fb / nb if nb else None, "boxes_skipped": sk,
+        "worst_relres": worst_rel, "worst_rel_diff_vs_fp64": worst_diff,
+        "data": "synthetic"}
+
+
 def cpu_baseline(G, index, budget_s=20.0):
     """The oracle (numpy/scipy restatement, FP64) on a bounded sample: the
     same workload family shrunk until one factor + solve fits the budget."""
@@ -195,6 +305,10 @@ def main():
     ap.add_argument("--top-schur", type=int, default=0, choices=(0, 32, 64),
                     help="arithmetic of the FP32 form's top-level LU Schur "
                          "updates (0: library default, FP32 matrix cores)")
+    ap.add_argument("--family", type=int, default=0,
+                    help="run only the smooth-curve family workload on this "
+                         "many test curves (64 held-out training curves) and "
+                         "print its summary line")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-predictor", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -214,6 +328,10 @@ def main():
 
     ki = importlib.import_module("kern-interp-dev_amd")
     G = importlib.import_module("kern-interp-dev_amd.geometry")
+    if args.family > 0:
+        print(json.dumps(family_batch(ki, G, args.family, tol=args.tol)),
+              flush=True)
+        return
     pb = G.config(args.config, args.scale)
     b = pb.boundary
     pde = ki.KI_LAPLACE if pb.pde == "laplace" else ki.KI_STOKES
