@@ -432,8 +432,15 @@ pchol_step(const CholDesc* __restrict__ desc, const double* __restrict__ g_all,
 //   lt  per box: CB x n tagged doubles, Lt(l - i0, j) = L(j, l)
 // At the end each chunk writes its diagonal to both halves of the double
 // buffer, where the next block (either form) and pchol_order read it.
+// REG: each wave also keeps its share of the block's L entries of its
+// lane's column in registers (Lr[t] = L(j, i0 + wid + 4 t), 32 doubles), so
+// the per-step sum reads no L from memory; the same terms are added in the
+// same order, so the factors stay bit-identical.  It holds 64 more VGPRs, so
+// fewer workgroups are resident and the host picks it only when every
+// chunk of the level still fits.
 constexpr int kCholChunk = 64;
 
+template <bool REG>
 __global__ void __launch_bounds__(256)
 pchol_coop(const CholDesc* __restrict__ desc, const int2* __restrict__ chunks,
            const double* __restrict__ g_all, double* __restrict__ ws_all,
@@ -464,6 +471,9 @@ pchol_coop(const CholDesc* __restrict__ desc, const int2* __restrict__ chunks,
   __shared__ double red[4][64];
   __shared__ double s_best;
   __shared__ int s_p;
+  double Lr[REG ? CB / 4 : 1];
+#pragma unroll
+  for (int t = 0; t < (REG ? CB / 4 : 1); t++) Lr[t] = 0.0;
   for (int i = i0; i < i1 && i < kend; i++) {
     const int s = i - i0;
     uint64_t* step = xch + (int64_t)s * wg_max * 3;
@@ -533,10 +543,18 @@ pchol_coop(const CholDesc* __restrict__ desc, const int2* __restrict__ chunks,
     __syncthreads();
     const bool live = j < n && dj >= 0.0 && j != p;
     double part = 0.0;
-    if (live)
-      part = strided_dot4<4>(
-          i0 + wid, i, [&](int l) { return L[j + (int64_t)n * l]; },
-          [&](int l) { return Lp[l - i0]; });
+    if (live) {
+      if constexpr (REG) {
+        // l = i0 + wid + 4 t ascending, as strided_dot4
+#pragma unroll
+        for (int t = 0; t < CB / 4; t++)
+          if (wid + 4 * t < s) part += Lr[t] * Lp[wid + 4 * t];
+      } else {
+        part = strided_dot4<4>(
+            i0 + wid, i, [&](int l) { return L[j + (int64_t)n * l]; },
+            [&](int l) { return Lp[l - i0]; });
+      }
+    }
     red[wid][lane] = part;
     __syncthreads();
     if (j < n) {
@@ -556,6 +574,13 @@ pchol_coop(const CholDesc* __restrict__ desc, const int2* __restrict__ chunks,
       if (wid == 0) {
         L[j + (int64_t)n * i] = v;
         TW::put(lt + 2 * ((int64_t)s * n + j), v, tag);
+      }
+      if constexpr (REG) {
+        if ((s & 3) == wid) {
+#pragma unroll
+          for (int t = 0; t < CB / 4; t++)
+            if (t == (s >> 2)) Lr[t] = v;
+        }
       }
     }
     __syncthreads();  // red and Lp are rewritten by the next step
@@ -710,6 +735,7 @@ void pchol_large(const std::vector<CholDesc>& h, const CholDesc* d_desc,
   for (int b = 0; b < nb; b++)
     for (int q = 0; q * kCholChunk < h[b].n; q++) chunk.push_back({b, q});
   const int nchunk = chunk.size();
+  bool coop_reg = false;  // the register form fits as well
   if (coop) {
     int dev = 0, ncu = 0, can = 0, per_cu = 0;
     KI_HIP(hipGetDevice(&dev));
@@ -718,8 +744,13 @@ void pchol_large(const std::vector<CholDesc>& h, const CholDesc* d_desc,
     KI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
                                  dev));
     KI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void*>(&pchol_coop), 256, 0));
+        &per_cu, reinterpret_cast<const void*>(&pchol_coop<false>), 256, 0));
+    int per_cu_reg = 0;
+    KI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu_reg, reinterpret_cast<const void*>(&pchol_coop<true>), 256,
+        0));
     coop = can && !coop_refused() && (int64_t)per_cu * ncu >= nchunk;
+    coop_reg = coop && (int64_t)per_cu_reg * ncu >= nchunk;
   }
   uint64_t* xch = nullptr;
   uint64_t* lt = nullptr;
@@ -767,8 +798,9 @@ void pchol_large(const std::vector<CholDesc>& h, const CholDesc* d_desc,
                       &state,        &i0,      &i1,       &wm,  &xch,
                       &d_lt_off,     &lt,      &tag,      &tol, &fault};
       const hipError_t e = hipLaunchCooperativeKernel(
-          reinterpret_cast<const void*>(&pchol_coop), dim3(nchunk),
-          dim3(256), args, 0, s);
+          coop_reg ? reinterpret_cast<const void*>(&pchol_coop<true>)
+                   : reinterpret_cast<const void*>(&pchol_coop<false>),
+          dim3(nchunk), dim3(256), args, 0, s);
       if (e == hipSuccess) {
         ran = true;
       } else {
