@@ -432,22 +432,31 @@ pchol_step(const CholDesc* __restrict__ desc, const double* __restrict__ g_all,
 //   lt  per box: CB x n tagged doubles, Lt(l - i0, j) = L(j, l)
 // At the end each chunk writes its diagonal to both halves of the double
 // buffer, where the next block (either form) and pchol_order read it.
-// REG: each wave also keeps its share of the block's L entries of its
-// lane's column in registers (Lr[t] = L(j, i0 + wid + 4 t), 32 doubles), so
-// the per-step sum reads no L from memory; the same terms are added in the
-// same order, so the factors stay bit-identical.  It holds 64 more VGPRs, so
-// fewer workgroups are resident and the host picks it only when every
-// chunk of the level still fits.
+// NR > 0: each wave also keeps its share of the block's L entries of its
+// lane's column on chip, Lr(t) = L(j, i0 + wid + 4 t) for t < CB / 4: slots
+// t < NR in registers, the rest in LDS (one slot row per thread).  The
+// per-step sum then reads no L from memory; the same terms are added in the
+// same order, so the factors stay bit-identical.  The slots cost registers
+// and LDS, so fewer workgroups are resident, and the host picks a form only
+// when every chunk of the level still fits.
 constexpr int kCholChunk = 64;
+// register slots of the hybrid form (the other CB / 4 - 18 in LDS): sized
+// so that five workgroups fit per CU (<= 96 VGPRs, <= 32 KB LDS)
+constexpr int kCholHybridRegs = 18;
 
-template <bool REG>
-__global__ void __launch_bounds__(256)
-pchol_coop(const CholDesc* __restrict__ desc, const int2* __restrict__ chunks,
-           const double* __restrict__ g_all, double* __restrict__ ws_all,
-           int* __restrict__ piv_all, int* __restrict__ state, int i0, int i1,
-           int wg_max, uint64_t* __restrict__ xch_all,
-           const int64_t* __restrict__ lt_off, uint64_t* __restrict__ lt_all,
-           uint32_t tag, double tol, int* __restrict__ fault) {
+#define KI_PCHOL_COOP_ARGS                                                   \
+  const CholDesc *__restrict__ desc, const int2 *__restrict__ chunks,        \
+      const double *__restrict__ g_all, double *__restrict__ ws_all,         \
+      int *__restrict__ piv_all, int *__restrict__ state, int i0, int i1,    \
+      int wg_max, uint64_t *__restrict__ xch_all,                            \
+      const int64_t *__restrict__ lt_off, uint64_t *__restrict__ lt_all,     \
+      uint32_t tag, double tol, int *__restrict__ fault
+#define KI_PCHOL_COOP_PASS                                                   \
+  desc, chunks, g_all, ws_all, piv_all, state, i0, i1, wg_max, xch_all,      \
+      lt_off, lt_all, tag, tol, fault
+
+template <int NR>
+__device__ __forceinline__ void pchol_coop_body(KI_PCHOL_COOP_ARGS) {
   typedef Tagged<double> TW;
   const int2 cb = chunks[blockIdx.x];
   const int box = cb.x, b = cb.y;
@@ -471,9 +480,11 @@ pchol_coop(const CholDesc* __restrict__ desc, const int2* __restrict__ chunks,
   __shared__ double red[4][64];
   __shared__ double s_best;
   __shared__ int s_p;
-  double Lr[REG ? CB / 4 : 1];
+  constexpr int NL = NR > 0 ? CB / 4 - NR : 0;  // slots in LDS
+  double Lr[NR > 0 ? NR : 1];
+  __shared__ double Ls[NL > 0 ? NL : 1][256];
 #pragma unroll
-  for (int t = 0; t < (REG ? CB / 4 : 1); t++) Lr[t] = 0.0;
+  for (int t = 0; t < (NR > 0 ? NR : 1); t++) Lr[t] = 0.0;
   for (int i = i0; i < i1 && i < kend; i++) {
     const int s = i - i0;
     uint64_t* step = xch + (int64_t)s * wg_max * 3;
@@ -544,11 +555,14 @@ pchol_coop(const CholDesc* __restrict__ desc, const int2* __restrict__ chunks,
     const bool live = j < n && dj >= 0.0 && j != p;
     double part = 0.0;
     if (live) {
-      if constexpr (REG) {
+      if constexpr (NR > 0) {
         // l = i0 + wid + 4 t ascending, as strided_dot4
 #pragma unroll
-        for (int t = 0; t < CB / 4; t++)
+        for (int t = 0; t < NR; t++)
           if (wid + 4 * t < s) part += Lr[t] * Lp[wid + 4 * t];
+#pragma unroll
+        for (int t = NR; t < CB / 4; t++)
+          if (wid + 4 * t < s) part += Ls[t - NR][tid] * Lp[wid + 4 * t];
       } else {
         part = strided_dot4<4>(
             i0 + wid, i, [&](int l) { return L[j + (int64_t)n * l]; },
@@ -575,11 +589,13 @@ pchol_coop(const CholDesc* __restrict__ desc, const int2* __restrict__ chunks,
         L[j + (int64_t)n * i] = v;
         TW::put(lt + 2 * ((int64_t)s * n + j), v, tag);
       }
-      if constexpr (REG) {
+      if constexpr (NR > 0) {
         if ((s & 3) == wid) {
+          const int ts = s >> 2;
+          if (ts >= NR) Ls[(ts - NR) % (NL > 0 ? NL : 1)][tid] = v;
 #pragma unroll
-          for (int t = 0; t < CB / 4; t++)
-            if (t == (s >> 2)) Lr[t] = v;
+          for (int t = 0; t < NR; t++)
+            if (t == ts) Lr[t] = v;
         }
       }
     }
@@ -589,6 +605,18 @@ pchol_coop(const CholDesc* __restrict__ desc, const int2* __restrict__ chunks,
     dg[j] = dj;
     dg[n + j] = dj;
   }
+}
+
+// The forms: L re-read from memory (57 VGPRs, eight workgroups per CU), all
+// slots in registers (130 VGPRs, three per CU), and the hybrid held to five
+// workgroups per CU.
+template <int NR>
+__global__ void __launch_bounds__(256) pchol_coop(KI_PCHOL_COOP_ARGS) {
+  pchol_coop_body<NR>(KI_PCHOL_COOP_PASS);
+}
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
+pchol_coop_hybrid(KI_PCHOL_COOP_ARGS) {
+  pchol_coop_body<kCholHybridRegs>(KI_PCHOL_COOP_PASS);
 }
 
 // Initialise the large-box state: d = diag(G) in the even buffer.
@@ -735,7 +763,7 @@ void pchol_large(const std::vector<CholDesc>& h, const CholDesc* d_desc,
   for (int b = 0; b < nb; b++)
     for (int q = 0; q * kCholChunk < h[b].n; q++) chunk.push_back({b, q});
   const int nchunk = chunk.size();
-  bool coop_reg = false;  // the register form fits as well
+  const void* coop_kernel = nullptr;  // the cooperative form launched
   if (coop) {
     int dev = 0, ncu = 0, can = 0, per_cu = 0;
     KI_HIP(hipGetDevice(&dev));
@@ -744,13 +772,25 @@ void pchol_large(const std::vector<CholDesc>& h, const CholDesc* d_desc,
     KI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
                                  dev));
     KI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void*>(&pchol_coop<false>), 256, 0));
-    int per_cu_reg = 0;
-    KI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu_reg, reinterpret_cast<const void*>(&pchol_coop<true>), 256,
-        0));
+        &per_cu, reinterpret_cast<const void*>(&pchol_coop<0>), 256, 0));
     coop = can && !coop_refused() && (int64_t)per_cu * ncu >= nchunk;
-    coop_reg = coop && (int64_t)per_cu_reg * ncu >= nchunk;
+    // the on-chip forms, most registers first, when all chunks still fit;
+    // KI_CHOL_COOP_FORM = 32 | 18 | 0 (register slots) allows only that form
+    const char* ff = std::getenv("KI_CHOL_COOP_FORM");
+    const int force = ff != nullptr ? std::atoi(ff) : -1;
+    const std::pair<int, const void*> forms[] = {
+        {CB / 4, reinterpret_cast<const void*>(&pchol_coop<CB / 4>)},
+        {kCholHybridRegs, reinterpret_cast<const void*>(&pchol_coop_hybrid)}};
+    for (const auto& f : forms) {
+      if (force >= 0 && force != f.first) continue;
+      int pc = 0;
+      KI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, f.second, 256,
+                                                          0));
+      if (coop && coop_kernel == nullptr && (int64_t)pc * ncu >= nchunk)
+        coop_kernel = f.second;
+    }
+    if (coop && coop_kernel == nullptr)
+      coop_kernel = reinterpret_cast<const void*>(&pchol_coop<0>);
   }
   uint64_t* xch = nullptr;
   uint64_t* lt = nullptr;
@@ -798,9 +838,7 @@ void pchol_large(const std::vector<CholDesc>& h, const CholDesc* d_desc,
                       &state,        &i0,      &i1,       &wm,  &xch,
                       &d_lt_off,     &lt,      &tag,      &tol, &fault};
       const hipError_t e = hipLaunchCooperativeKernel(
-          coop_reg ? reinterpret_cast<const void*>(&pchol_coop<true>)
-                   : reinterpret_cast<const void*>(&pchol_coop<false>),
-          dim3(nchunk), dim3(256), args, 0, s);
+          coop_kernel, dim3(nchunk), dim3(256), args, 0, s);
       if (e == hipSuccess) {
         ran = true;
       } else {

@@ -354,12 +354,19 @@ def test_gram_and_householder_ids_agree(name):
     assert same == len(hh)
 
 
+@pytest.mark.parametrize("form", ["", "32", "18", "0"])
 @pytest.mark.parametrize("precision", [64, 32])
-def test_cooperative_and_per_step_cholesky_agree(precision):
+def test_cooperative_and_per_step_cholesky_agree(precision, form,
+                                                 monkeypatch):
     """The large-box pivoted Cholesky of the Gram ID, one block of pivot
     steps per cooperative launch (chol_coop=1) or one launch per step
     (chol_coop=0): same expressions in the same order, so the same ranks,
-    the same skeletons and bit-identical solutions."""
+    the same skeletons and bit-identical solutions.  Each cooperative form
+    is forced in turn (KI_CHOL_COOP_FORM: the block's L entries all in
+    registers, split between registers and LDS, or re-read from memory;
+    "" = the library's choice)."""
+    if form:
+        monkeypatch.setenv("KI_CHOL_COOP_FORM", form)
     b, pde, kw, leaf, prm, ctx, okern, otree = setup("stokes3d_ellipsoid")
     rng = np.random.default_rng(12)
     rhs = rng.normal(size=ctx.ndof)
