@@ -727,6 +727,54 @@ void pchol_large(const std::vector<CholDesc>& h, const CholDesc* d_desc,
                  double tol, bool coop, hipStream_t s) {
   const int nb = h.size();
   if (nb == 0) return;
+  // A level with more chunks than any cooperative form keeps resident: its
+  // boxes run in groups that each fit the hybrid form, one call per group
+  // (the boxes are independent and keep their slots, so the results are
+  // those of the one-call forms).  KI_CHOL_COOP_FORM turns this off.
+  if (coop && nb > 1 && std::getenv("KI_CHOL_COOP_FORM") == nullptr &&
+      !coop_refused()) {
+    int dev = 0, ncu = 0, can = 0, pc_mem = 0, pc_hyb = 0;
+    KI_HIP(hipGetDevice(&dev));
+    KI_HIP(hipDeviceGetAttribute(&can, hipDeviceAttributeCooperativeLaunch,
+                                 dev));
+    KI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                                 dev));
+    KI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &pc_mem, reinterpret_cast<const void*>(&pchol_coop<0>), 256, 0));
+    KI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &pc_hyb, reinterpret_cast<const void*>(&pchol_coop_hybrid), 256, 0));
+    int64_t total = 0;
+    for (auto& x : h) total += (x.n + kCholChunk - 1) / kCholChunk;
+    const int64_t cap = (int64_t)pc_hyb * ncu;
+    if (can && (int64_t)pc_mem * ncu < total && cap > 0) {
+      std::vector<std::vector<CholDesc>> groups(1);
+      int64_t used = 0;
+      bool fits = true;
+      for (auto& x : h) {
+        const int64_t c = (x.n + kCholChunk - 1) / kCholChunk;
+        fits = fits && c <= cap;
+        if (used + c > cap && !groups.back().empty()) {
+          groups.emplace_back();
+          used = 0;
+        }
+        groups.back().push_back(x);
+        used += c;
+      }
+      if (fits && groups.size() > 1) {
+        for (auto& gp : groups) {
+          CholDesc* dgd = nullptr;
+          KI_HIP(hipMallocAsync((void**)&dgd, sizeof(CholDesc) * gp.size(),
+                                s));
+          KI_HIP(hipMemcpyAsync(dgd, gp.data(), sizeof(CholDesc) * gp.size(),
+                                hipMemcpyHostToDevice, s));
+          // returns after a stream synchronize: gp outlives the copy
+          pchol_large(gp, dgd, g, ws, piv, state, rank, tol, true, s);
+          KI_HIP(hipFreeAsync(dgd, s));
+        }
+        return;
+      }
+    }
+  }
   int maxn = 0, maxsteps = 0;
   for (auto& x : h) {
     maxn = std::max(maxn, x.n);
@@ -818,13 +866,16 @@ void pchol_large(const std::vector<CholDesc>& h, const CholDesc* d_desc,
     fault = reinterpret_cast<int*>(xch + xw - 1);
     KI_HIP(hipStreamSynchronize(s));  // lo dies with this scope
   }
-  std::vector<int> hst(4 * (size_t)nb);
+  // state is indexed by each box's slot in the level (not its index in h)
+  int nslot = 0;
+  for (auto& x : h) nslot = std::max(nslot, x.slot + 1);
+  std::vector<int> hst(4 * (size_t)nslot);
   auto all_done = [&]() {
-    KI_HIP(hipMemcpyAsync(hst.data(), state, sizeof(int) * 4 * nb,
+    KI_HIP(hipMemcpyAsync(hst.data(), state, sizeof(int) * 4 * nslot,
                           hipMemcpyDeviceToHost, s));
     KI_HIP(hipStreamSynchronize(s));
     bool all = true;
-    for (int b = 0; b < nb; b++) all = all && hst[4 * b + 1];
+    for (auto& x : h) all = all && hst[4 * (size_t)x.slot + 1];
     return all;
   };
   uint32_t tag = 0;

@@ -53,3 +53,31 @@ def test_full_size_fp32_and_fp64_solutions_agree(index):
           (pb.name, diff, err), flush=True)
     assert diff <= 1e-10
     assert err <= FIELD_BAR[index]
+
+
+def test_full_size_cooperative_cholesky_is_bit_identical():
+    """The 3D Stokes ellipsoid at full size exercises every form of the
+    cooperative pivoted Cholesky: registers (level 1), registers + LDS
+    (level 2) and box groups that each fit (levels 3 and 4, whose chunks
+    exceed what one launch keeps resident).  Against one launch per pivot
+    step (chol_coop = 0) the FP32 factorization must give the same top size
+    and a bit-identical direct solve."""
+    pb = G.config(4)
+    b = pb.boundary
+    print("[case] full size %s, cooperative vs per-step Cholesky" % pb.name,
+          flush=True)
+    ctx = ki.Context()
+    ctx.set_geometry(b, ki.KI_STOKES)
+    ctx.build_tree(pb.max_leaf_dofs)
+    out = []
+    for coop in (1, 0):
+        ctx.skeletonize(ki.default_params(
+            id_tol=pb.id_tol, proxy_points=pb.proxy_points,
+            proxy_ratio=pb.proxy_ratio, proxy_kind=1, precision=32,
+            chol_coop=coop))
+        out.append((ctx.stats()["top_size"], ctx.solve(pb.rhs)))
+    (t1, x1), (t0, x0) = out
+    print("[full] cooperative vs per-step: top %d / %d, max |dx| %.2e" %
+          (t1, t0, np.abs(x1 - x0).max()), flush=True)
+    assert t1 == t0
+    assert np.array_equal(x1, x0)
