@@ -206,7 +206,8 @@ struct SliceReader {
 };
 
 // SYM: C is symmetric (m = n) and so is the product (B = A, TB = !TA): only
-// the upper tiles run, each writing its block and the mirrored one.
+// the tiles on and above the diagonal run, and the lower triangle of C is
+// left as it was (its readers take entry (i, j) from (j, i)).
 // TM: the type the products run in.  Operands are read in their storage type
 // T and rounded to TM when staged in LDS; the TM accumulators are added into
 // C of type TC (FP32 matrix-core Schur product into an FP64 matrix: T = TM =
@@ -302,7 +303,6 @@ gemm128_kernel(const GemmDesc* __restrict__ desc, const T* __restrict__ a_all,
         const TC x = (TC)acc[a][b][v];
         const TC nv = beta == (TC)0 ? alpha * x : alpha * x + beta * cv[b][v];
         C[gi + (int64_t)d.ldc * gj] = nv;
-        if (SYM && i0 != j0) C[gj + (int64_t)d.ldc * gi] = nv;
       }
     }
   }

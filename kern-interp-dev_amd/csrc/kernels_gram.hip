@@ -12,7 +12,8 @@
 // the k streaming passes over A of a BLAS-2 QR.
 //
 //   gram_kernel   G = A^T A for every box of a level, FP64 accumulation,
-//                 upper-triangle tiles only (mirrored)
+//                 upper-triangle tiles only (the lower triangle is not
+//                 written; readers take entry (j, p) from the upper one)
 //   pchol_wg      left-looking pivoted Cholesky, one workgroup per box
 //   pchol_step    one pivot step for a few large boxes, many workgroups
 //                 per box (the host loops over steps)
@@ -48,6 +49,14 @@ __device__ __forceinline__ double strided_dot4(int l0, int l1, FA a, FB b) {
       if (lb + 4 * u < l1) part += va[u] * vb[u];
   }
   return part;
+}
+
+// Entry (j, p) of a symmetric n x n matrix stored column-major whose upper
+// triangle (row <= column) alone is kept up to date: the Gram product and
+// the Schur updates write only their upper tiles.
+__device__ __forceinline__ double sym_upper(const double* __restrict__ G,
+                                            int n, int j, int p) {
+  return j <= p ? G[j + (int64_t)n * p] : G[p + (int64_t)n * j];
 }
 
 constexpr int GT = 64, GK = 32;
@@ -117,7 +126,6 @@ gram_kernel(const GramDesc* __restrict__ desc, const TI* __restrict__ a_all,
         const int gj = j0 + wc + 16 * b + Mfma<double>::row(lane, v);
         if (gi >= d.n || gj >= d.n) continue;
         G[gi + (int64_t)d.n * gj] = acc[a][b][v];
-        G[gj + (int64_t)d.n * gi] = acc[a][b][v];
       }
     }
   }
@@ -212,7 +220,6 @@ gram128_kernel(const GramDesc* __restrict__ desc, const TI* __restrict__ a_all,
         const int gj = j0 + wc + 16 * b + Mfma<double>::row(lane, v);
         if (gi >= n || gj >= n) continue;
         G[gi + (int64_t)n * gj] = acc[a][b][v];
-        G[gj + (int64_t)n * gi] = acc[a][b][v];
       }
     }
   }
@@ -317,7 +324,7 @@ pchol_wg(const CholDesc* __restrict__ desc, double* __restrict__ g_all,
         L[j + (int64_t)n * i] = rii;
         continue;
       }
-      double sm = G[j + (int64_t)n * p];
+      double sm = sym_upper(G, n, j, p);
       for (int l = 0; l < i; l++)
         sm -= L[j + (int64_t)n * l] * L[p + (int64_t)n * l];
       const double v = sm * inv;
@@ -409,7 +416,7 @@ pchol_step(const CholDesc* __restrict__ desc, const double* __restrict__ g_all,
     dout[j] = -1.0;
     return;
   }
-  const double sm = G[j + (int64_t)n * p] -
+  const double sm = sym_upper(G, n, j, p) -
                     ((red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]));
   const double v = sm / rii;
   L[j + (int64_t)n * i] = v;
@@ -580,7 +587,7 @@ __device__ __forceinline__ void pchol_coop_body(KI_PCHOL_COOP_ARGS) {
         dj = -1.0;
       } else {
         const double sm =
-            G[j + (int64_t)n * p] -
+            sym_upper(G, n, j, p) -
             ((red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]));
         v = sm / rii;
         dj = fmax(dj - v * v, 0.0);
@@ -913,9 +920,9 @@ void pchol_large(const std::vector<CholDesc>& h, const CholDesc* d_desc,
       stop = all_done();
     }
     if (stop || b1 >= maxsteps) break;
-    // S -= L_blk L_blk^T for every box in one launch: S stays exactly
-    // symmetric, so only the upper tiles are computed and each is mirrored
-    // (half the flops of the full product)
+    // S -= L_blk L_blk^T for every box in one launch: S is symmetric, so
+    // only the tiles on and above the diagonal are computed and kept (half
+    // the flops of the full product; the steps read S through sym_upper)
     launch_sym_update<double>(d_g + (size_t)bk * nb, nb, maxn, ws, g, s);
   }
   if (xch != nullptr) {
